@@ -1,0 +1,134 @@
+/*
+ * xmatch.h -- C ABI of the MI355X zone-algorithm crossmatch (libxmatch_amd.so).
+ *
+ * Catalog segments of (id, ra, dec) go in; (idA, idB, distance) pairs within a
+ * search radius come out.  The method is the zone algorithm of BoostXmatch
+ * (matthiaslee/Xmatch): declination zones of height h, sources sorted by
+ * (zone, ra), each A zone joined with B zones z-nz..z+nz over an ra window
+ * widened by alpha(dec), then an exact chord test.
+ *
+ * Every entry point returns xm_status.  Nothing is truncated: when the result
+ * buffer overflows the library grows it and re-runs the join.  Plain pointers
+ * and sizes only; device pointers are HIP device addresses on the context's GPU.
+ *
+ * Threading: calls on one xm_ctx are serialised by an internal mutex; all device
+ * work of a context runs on its one HIP stream.  Different contexts are
+ * independent.
+ */
+#ifndef XMATCH_H
+#define XMATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XM_ABI_VERSION 1
+
+typedef enum {
+    XM_OK = 0,
+    XM_EINVAL = 1,    /* bad argument or bad input record (NaN, dec outside [-90,90], ra outside [0,360)) */
+    XM_ENOMEM = 2,    /* device or host allocation failed */
+    XM_EDEVICE = 3,   /* no usable GPU, or a HIP call failed */
+    XM_EINTERNAL = 4
+} xm_status;
+
+typedef enum {
+    XM_FP64_PLAIN = 0,              /* FP64 unit vectors, FP64 chord test on every candidate */
+    XM_FP32_FILTER_FP64_RECHECK = 1 /* FP32 filter with a guard band, FP64 recheck in the band */
+} xm_precision;
+
+/* One input record, 24 bytes, packed little-endian.
+ * Replaces xmatch::Obj (BoostXmatch/BoostXmatch/Obj.h:21-25). */
+typedef struct {
+    int64_t id;
+    double ra_deg;
+    double dec_deg;
+} xm_obj;
+
+typedef struct xm_ctx xm_ctx;
+typedef struct xm_catalog xm_catalog;
+typedef struct xm_result xm_result;
+
+/* Work and timing figures of one xm_match call. */
+typedef struct {
+    uint64_t n_matches;      /* pairs in the result */
+    uint64_t n_chunks;       /* A row chunks (one wavefront each) */
+    uint64_t n_retries;      /* result-buffer overflows that forced a re-run */
+    uint64_t band_hits;      /* FP32 filter only: pairs inside the guard band (FP64 rechecked) */
+    double join_ms;          /* device time of the join (HIP events), including re-runs */
+} xm_match_stats;
+
+/* ---- context: replaces CudaManager/CudaContext (CudaManager.cpp:8-65,
+ *      CudaContext.cpp:5-27) with one explicit per-device handle. */
+xm_status xm_ctx_create(int device, xm_ctx **out);
+void xm_ctx_destroy(xm_ctx *ctx);
+/* Message of the last failed call on this context (never NULL). */
+const char *xm_last_error_message(const xm_ctx *ctx);
+/* Per-kernel device timing with HIP events on the context stream (0 = off). */
+xm_status xm_ctx_set_timing(xm_ctx *ctx, int enable);
+/* Sum and count of recorded durations for kernel `name` since the last reset. */
+xm_status xm_ctx_kernel_time(xm_ctx *ctx, const char *name, double *total_ms, uint64_t *count);
+xm_status xm_ctx_reset_timing(xm_ctx *ctx);
+/* Wait for all device work of the context. */
+xm_status xm_ctx_synchronize(xm_ctx *ctx);
+/* Device malloc/free through the context's stream-ordered pool (for callers that
+ * stage records on the device). */
+xm_status xm_device_alloc(xm_ctx *ctx, size_t bytes, void **dptr);
+xm_status xm_device_free(xm_ctx *ctx, void *dptr);
+xm_status xm_memcpy_h2d(xm_ctx *ctx, void *dst, const void *src, size_t bytes);
+xm_status xm_memcpy_d2h(xm_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- segment preparation: replaces Sorter::Sort (BoostXmatch/BoostXmatch/
+ *      Sorter.cu:141-185) and Segment::Load's in-memory result (Segment.cpp:14-22).
+ * Builds a device-resident, zone-sorted, zone-indexed catalog.  zone_height_deg
+ * is the reference's zh_deg (BoostXmatch.cpp:158,271: default = radius).
+ * The host variant copies the records to the device first. */
+xm_status xm_catalog_create(xm_ctx *ctx, const xm_obj *records, uint64_t n,
+                            double zone_height_deg, xm_catalog **out);
+xm_status xm_catalog_create_device(xm_ctx *ctx, const xm_obj *d_records, uint64_t n,
+                                   double zone_height_deg, xm_catalog **out);
+void xm_catalog_destroy(xm_catalog *cat);
+/* n, zone height, occupied zone range [zmin, zmax] (zmin > zmax when empty). */
+xm_status xm_catalog_info(const xm_catalog *cat, uint64_t *n, double *zone_height_deg,
+                          int32_t *zmin, int32_t *zmax);
+/* Sorted-order export to host arrays of length n (any pointer may be NULL):
+ * ids, FP64 unit vectors, zone ids.  For tests and tooling. */
+xm_status xm_catalog_export(xm_ctx *ctx, const xm_catalog *cat, int64_t *id, double *x,
+                            double *y, double *z, int32_t *zone);
+
+/* ---- join: replaces Worker::Match (BoostXmatch/BoostXmatch/Worker.cu:243-428)
+ *      and xmatch_kernel (Worker.cu:65-241).
+ * radius_deg is Job::sr_deg (Job.h:21).  capacity_hint sizes the first result
+ * buffer (0 = 2*max(nA,nB), as the reference's maxout default, Worker.cu:295-300);
+ * overflow never truncates.  stats may be NULL. */
+xm_status xm_match(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b, double radius_deg,
+                   xm_precision precision, uint64_t capacity_hint, xm_result **out,
+                   xm_match_stats *stats);
+xm_status xm_result_count(const xm_result *res, uint64_t *n);
+/* Device pointers to the n result triples (valid until xm_result_destroy). */
+xm_status xm_result_device_ptrs(const xm_result *res, const int64_t **id_a,
+                                const int64_t **id_b, const double **dist_arcsec);
+/* Copy the result to host arrays of length >= count (pointers may be NULL). */
+xm_status xm_result_copy(xm_ctx *ctx, const xm_result *res, int64_t *id_a, int64_t *id_b,
+                         double *dist_arcsec);
+void xm_result_destroy(xm_result *res);
+
+/* ---- work counts, computed on the host from the catalogs' sorted keys and
+ * zone tables (never from device counters):
+ *   w_ref  = sum_z nA(z) * sum_{|z'-z|<=nz} nB(z')        (reference dense tests)
+ *   w_cand = #{(a,b): |za-zb|<=nz, cyclic |dra| <= alpha(za)} (pairs reaching the chord)
+ *   w_tile = tests the pair kernel executes under its chunking */
+xm_status xm_work_counts(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
+                         double radius_deg, uint64_t *w_ref, uint64_t *w_cand,
+                         uint64_t *w_tile);
+
+/* ---- version */
+int xm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XMATCH_H */

@@ -1,0 +1,239 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+golden fixtures.  The bar: identical (idA, idB) sets (bit-exact), distances within
+1e-6 arcsec, FP64 unit vectors bit-identical to the oracle's."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import xmatch_amd as xm
+from oracle import oracle as O
+from xmatch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DIST_TOL_ARCSEC = 1e-6
+
+
+def gpu_join(ctx, a, b, r, h=None, capacity_hint=0, precision=xm.FP64_PLAIN):
+    h = r if h is None else h
+    ca = ctx.catalog(a, h)
+    cb = ctx.catalog(b, h)
+    res = ctx.match(ca, cb, r, precision=precision, capacity_hint=capacity_hint)
+    out = res.to_host()
+    stats = res.stats.as_dict()
+    res.close(); ca.close(); cb.close()
+    return out, stats
+
+
+def assert_same(got, ref):
+    ia, ib, d = got
+    ra, rb, rd = ref
+    assert len(ia) == len(ra), (len(ia), len(ra))
+    assert np.array_equal(ia, ra) and np.array_equal(ib, rb)
+    if len(d):
+        assert np.abs(d - rd).max() <= DIST_TOL_ARCSEC
+
+
+def test_unit_vectors_and_zones_bit_exact(gpu_ctx):
+    A, _, r = synth.config_small_patch(seed=1)
+    special = synth.make_objs(np.arange(10**7, 10**7 + 8),
+                              [0.0, 90.0, 180.0, 270.0, 359.999999999, 12.5, 0.0, 45.0],
+                              [0.0, 0.0, 0.0, 0.0, 0.0, 90.0, -90.0, -45.0])
+    recs = np.concatenate([A, special])
+    cat = gpu_ctx.catalog(recs, r)
+    e = cat.export()
+    cat.close()
+    o = np.argsort(recs["id"])
+    p = np.argsort(e["id"])
+    assert np.array_equal(e["id"][p], recs["id"][o])
+    x, y, z = O.unitvec(recs["ra"][o], recs["dec"][o])
+    assert np.array_equal(e["x"][p].view(np.int64), x.view(np.int64))
+    assert np.array_equal(e["y"][p].view(np.int64), y.view(np.int64))
+    assert np.array_equal(e["z"][p].view(np.int64), z.view(np.int64))
+    assert np.array_equal(e["zone"][p], O.zone(recs["dec"][o], r))
+
+
+def test_sorted_by_zone_then_ra(gpu_ctx):
+    A, _, r = synth.config_small_patch(seed=3)
+    cat = gpu_ctx.catalog(A, r)
+    e = cat.export()
+    info = cat.info()
+    cat.close()
+    assert np.all(np.diff(e["zone"]) >= 0)
+    assert info["zmin"] == e["zone"][0] and info["zmax"] == e["zone"][-1]
+    ra = dict(zip(A["id"].tolist(), A["ra"].tolist()))
+    ras = np.array([ra[i] for i in e["id"]])
+    same = np.diff(e["zone"]) == 0
+    assert np.all(np.diff(ras)[same] >= -360.0 / 2**24)
+
+
+@pytest.mark.parametrize("r_arcsec", [1, 2, 5])
+def test_selfjoin_sample_data_golden(gpu_ctx, r_arcsec):
+    a = synth.read_idradec(os.path.join(GOLD, "test.idradec"))
+    g = np.load(os.path.join(GOLD, "selfjoin_r%d.npz" % r_arcsec))
+    got, _ = gpu_join(gpu_ctx, a, a, r_arcsec / 3600.0)
+    assert_same(got, (g["ida"], g["idb"], g["dist_arcsec"]))
+    ident = got[0] == got[1]
+    assert ident.sum() == 10000 and np.all(got[2][ident] == 0.0)
+
+
+def test_small_patch_vs_oracle_and_golden(gpu_ctx):
+    A, B, r = synth.config_small_patch(seed=1)
+    got, stats = gpu_join(gpu_ctx, A, B, r)
+    ref, _ = O.match(A, B, r)
+    assert_same(got, ref)
+    with open(os.path.join(GOLD, "small_patch_seed1.json")) as f:
+        g = json.load(f)
+    h = hashlib.sha256()
+    h.update(got[0].astype("<i8").tobytes())
+    h.update(got[1].astype("<i8").tobytes())
+    assert h.hexdigest() == g["sha256_pairs"] and stats["n_matches"] == g["count"]
+    assert stats["n_retries"] == 0
+
+
+def test_overflow_retry_never_truncates(gpu_ctx):
+    A, B, r = synth.config_small_patch(seed=1)
+    got, stats = gpu_join(gpu_ctx, A, B, r, capacity_hint=1000)
+    assert stats["n_retries"] >= 1
+    ref, _ = O.match(A, B, r)
+    assert_same(got, ref)
+
+
+@pytest.mark.parametrize("where", ["seam", "north_pole", "south_pole", "dense_cluster", "mid"])
+def test_edge_geometry_vs_allpairs(gpu_ctx, where):
+    rng = np.random.default_rng(abs(hash(where)) % 2**32)
+    n = 4000
+    if where == "seam":
+        ra = np.where(rng.random(n) < 0.5, rng.uniform(0, 0.01, n),
+                      rng.uniform(359.99, 360, n) % 360)
+        dec = rng.uniform(-0.02, 0.02, n)
+    elif where == "north_pole":
+        ra, dec = synth.uniform_patch(rng, n, 0, 360, 89.97, 90.0)
+    elif where == "south_pole":
+        ra, dec = synth.uniform_patch(rng, n, 0, 360, -90.0, -89.97)
+    elif where == "dense_cluster":  # everything in two zones
+        ra, dec = synth.uniform_patch(rng, n, 150.0, 150.003, 20.0, 20.0003)
+    else:
+        ra, dec = synth.uniform_patch(rng, n, 200, 200.04, 45, 45.03)
+    a = synth.make_objs(np.arange(n), ra, dec)
+    bra, bdec = synth.perturb(rng, ra, dec, 2.0)
+    b = synth.make_objs(10**9 + np.arange(n), bra, bdec)
+    if where == "north_pole":
+        a["dec"][:3] = 90.0
+        b["dec"][:3] = 90.0
+    for r in (1.5 / 3600, 4 / 3600):
+        got, _ = gpu_join(gpu_ctx, a, b, r)
+        assert_same(got, O.brute(a, b, r))
+
+
+def test_zone_height_below_radius(gpu_ctx):
+    rng = np.random.default_rng(11)
+    ra, dec = synth.uniform_patch(rng, 5000, 30, 30.02, 10, 10.02)
+    a = synth.make_objs(np.arange(5000), ra, dec)
+    ra, dec = synth.uniform_patch(rng, 5000, 30, 30.02, 10, 10.02)
+    b = synth.make_objs(10**9 + np.arange(5000), ra, dec)
+    r = 6 / 3600
+    got, _ = gpu_join(gpu_ctx, a, b, r, h=r / 3)
+    assert_same(got, O.brute(a, b, r, zone_height_deg=r / 3))
+
+
+def test_boundary_pairs_match_oracle_exactly(gpu_ctx):
+    # pairs at distance r(1 + eps) with eps down to 2^-50: rounding decides these,
+    # so only an identical FP64 evaluation gives the oracle's set
+    rng = np.random.default_rng(5)
+    r = 1.0 / 3600
+    recs_a, recs_b = [], []
+    k = 0
+    for dec0 in (0.0, 0.3, 37.0, -62.0, 88.0):
+        for ra0 in (0.0, 45.0, 180.0, 359.9999999):
+            for e in (-2.0**-30, -2.0**-40, -2.0**-45, -2.0**-50, 0.0, 2.0**-50, 2.0**-45,
+                      2.0**-40, 2.0**-30):
+                dist = r * (1 + e)
+                c = np.cos(np.radians(dec0))
+                dra = np.degrees(2 * np.arcsin(np.sin(np.radians(dist) / 2) / c))
+                for (pa, pb) in (((ra0, dec0), (ra0, dec0 + dist)),
+                                 ((ra0, dec0), ((ra0 + dra) % 360, dec0)),
+                                 ((ra0, dec0), ((ra0 - dra) % 360, dec0))):
+                    recs_a.append((k, pa[0], pa[1]))
+                    recs_b.append((10**9 + k, pb[0], pb[1]))
+                    k += 1
+    a = np.array(recs_a, dtype=synth.OBJ_DTYPE)
+    b = np.array(recs_b, dtype=synth.OBJ_DTYPE)
+    b["dec"] = np.clip(b["dec"], -90, 90)
+    got, _ = gpu_join(gpu_ctx, a, b, r)
+    assert_same(got, O.match(a, b, r)[0])
+    assert_same(got, O.brute(a, b, r))
+
+
+def test_empty_and_single(gpu_ctx):
+    one = synth.make_objs([7], [10.0], [10.0])
+    other = synth.make_objs([8], [10.0], [10.0 + 0.5 / 3600])
+    empty = one[:0]
+    r = 1 / 3600
+    got, _ = gpu_join(gpu_ctx, empty, one, r)
+    assert len(got[0]) == 0
+    got, _ = gpu_join(gpu_ctx, one, empty, r)
+    assert len(got[0]) == 0
+    got, _ = gpu_join(gpu_ctx, empty, empty, r)
+    assert len(got[0]) == 0
+    got, _ = gpu_join(gpu_ctx, one, other, r)
+    assert got[0].tolist() == [7] and got[1].tolist() == [8]
+    assert abs(got[2][0] - 0.5) < 1e-6
+
+
+@pytest.mark.parametrize("bad", ["nan_ra", "nan_dec", "dec_high", "dec_low", "ra_360", "ra_neg"])
+def test_invalid_records_rejected(gpu_ctx, bad):
+    a = synth.make_objs([1, 2, 3], [10.0, 20.0, 30.0], [0.0, 1.0, 2.0])
+    field, val = {"nan_ra": ("ra", np.nan), "nan_dec": ("dec", np.nan),
+                  "dec_high": ("dec", 90.0001), "dec_low": ("dec", -90.5),
+                  "ra_360": ("ra", 360.0), "ra_neg": ("ra", -1e-9)}[bad]
+    a[field][1] = val
+    with pytest.raises(xm.XmatchError) as ei:
+        gpu_ctx.catalog(a, 1 / 3600)
+    assert ei.value.status == xm.XM_EINVAL
+    # the context stays usable after the error
+    got, _ = gpu_join(gpu_ctx, a[:1], a[:1], 1 / 3600)
+    assert len(got[0]) == 1
+
+
+def test_bad_join_arguments(gpu_ctx):
+    a = synth.make_objs([1], [10.0], [0.0])
+    ca = gpu_ctx.catalog(a, 1 / 3600)
+    cb = gpu_ctx.catalog(a, 2 / 3600)
+    with pytest.raises(xm.XmatchError) as ei:
+        gpu_ctx.match(ca, cb, 1 / 3600)
+    assert ei.value.status == xm.XM_EINVAL
+    with pytest.raises(xm.XmatchError):
+        gpu_ctx.match(ca, ca, -1.0)
+    ca.close(); cb.close()
+
+
+def test_segmentation_invariance(gpu_ctx):
+    A, B, r = synth.config_small_patch(seed=5, n_a=30000, n_b=30000)
+    whole, _ = gpu_join(gpu_ctx, A, B, r)
+    parts = []
+    for sa in np.array_split(A, 3):
+        for sb in np.array_split(B, 2):
+            parts.append(gpu_join(gpu_ctx, sa, sb, r)[0])
+    x = np.concatenate([p[0] for p in parts])
+    y = np.concatenate([p[1] for p in parts])
+    d = np.concatenate([p[2] for p in parts])
+    o = np.lexsort((y, x))
+    assert_same((x[o], y[o], d[o]), whole)
+
+
+def test_full_sky_2m_vs_oracle(gpu_ctx):
+    A, B, r = synth.config_full_sky(seed=2, n_a=2_000_000, n_b=2_000_000)
+    got, stats = gpu_join(gpu_ctx, A, B, r)
+    ref, cnt = O.match(A, B, r, mode=0)
+    assert_same(got, ref)
+    ca, cb = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
+    w = gpu_ctx.work_counts(ca, cb, r)
+    ca.close(); cb.close()
+    assert w["w_ref"] == cnt["w_ref"]
+    assert abs(w["w_cand"] - cnt["w_cand"]) <= 1e-4 * cnt["w_cand"] + 10
+    assert w["w_tile"] >= w["w_cand"] * 0.99

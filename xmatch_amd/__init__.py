@@ -1,0 +1,278 @@
+"""xmatch_amd -- MI355X-native zone-algorithm crossmatch (ctypes binding of the C ABI).
+
+The product is libxmatch_amd.so (HIP kernels for gfx950 behind include/xmatch.h).
+This module is a thin ctypes layer over that ABI for tests, the bench and Python
+callers.  It has no CPU fallback: if the library is missing or no GPU is usable,
+calls raise.
+
+Reference interface mirrored (paths relative to the reference repository):
+  Segment + Sorter::Sort   BoostXmatch/BoostXmatch/Segment.h:22-41, Sorter.cu:141-185
+                           -> Catalog
+  Job + Worker::Match      Job.h:16-29, Worker.cu:243-428 -> Context.match
+  JobManager cross product JobManager.cpp:5-13           -> xmatch_amd.jobs
+"""
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+from .synth import OBJ_DTYPE, ARCSEC  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxmatch_amd.so")
+
+XM_OK, XM_EINVAL, XM_ENOMEM, XM_EDEVICE, XM_EINTERNAL = range(5)
+FP64_PLAIN = 0
+FP32_FILTER_FP64_RECHECK = 1
+_STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_EINTERNAL"}
+
+# every symbol declared in include/xmatch.h
+EXPORTED = [
+    "xm_abi_version", "xm_ctx_create", "xm_ctx_destroy", "xm_last_error_message",
+    "xm_ctx_set_timing", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
+    "xm_device_alloc", "xm_device_free", "xm_memcpy_h2d", "xm_memcpy_d2h",
+    "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
+    "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
+    "xm_result_copy", "xm_result_destroy", "xm_work_counts",
+]
+
+
+class XmatchError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (_STATUS.get(status, status), message))
+        self.status = status
+
+
+class MatchStats(ctypes.Structure):
+    _fields_ = [("n_matches", ctypes.c_uint64), ("n_chunks", ctypes.c_uint64),
+                ("n_retries", ctypes.c_uint64), ("band_hits", ctypes.c_uint64),
+                ("join_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libxmatch_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("%s is not built; run __graft_entry__.build() "
+                              "(hipcc --offload-arch=gfx950)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i32, dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
+        pvp = ctypes.POINTER(ctypes.c_void_p)
+        sigs = {
+            "xm_abi_version": ([], ctypes.c_int),
+            "xm_ctx_create": ([ctypes.c_int, pvp], ctypes.c_int),
+            "xm_ctx_destroy": ([vp], None),
+            "xm_last_error_message": ([vp], ctypes.c_char_p),
+            "xm_ctx_set_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "xm_ctx_kernel_time": ([vp, ctypes.c_char_p, ctypes.POINTER(dbl),
+                                    ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_ctx_reset_timing": ([vp], ctypes.c_int),
+            "xm_ctx_synchronize": ([vp], ctypes.c_int),
+            "xm_device_alloc": ([vp, ctypes.c_size_t, pvp], ctypes.c_int),
+            "xm_device_free": ([vp, vp], ctypes.c_int),
+            "xm_memcpy_h2d": ([vp, vp, vp, ctypes.c_size_t], ctypes.c_int),
+            "xm_memcpy_d2h": ([vp, vp, vp, ctypes.c_size_t], ctypes.c_int),
+            "xm_catalog_create": ([vp, vp, u64, dbl, pvp], ctypes.c_int),
+            "xm_catalog_create_device": ([vp, vp, u64, dbl, pvp], ctypes.c_int),
+            "xm_catalog_destroy": ([vp], None),
+            "xm_catalog_info": ([vp, ctypes.POINTER(u64), ctypes.POINTER(dbl),
+                                 ctypes.POINTER(i32), ctypes.POINTER(i32)], ctypes.c_int),
+            "xm_catalog_export": ([vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "xm_match": ([vp, vp, vp, dbl, ctypes.c_int, u64, pvp, ctypes.POINTER(MatchStats)],
+                         ctypes.c_int),
+            "xm_result_count": ([vp, ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_result_device_ptrs": ([vp, pvp, pvp, pvp], ctypes.c_int),
+            "xm_result_copy": ([vp, vp, vp, vp, vp], ctypes.c_int),
+            "xm_result_destroy": ([vp], None),
+            "xm_work_counts": ([vp, vp, vp, dbl, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                ctypes.POINTER(u64)], ctypes.c_int),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One GPU: replaces the reference's CudaManager/CudaContext pair."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()
+        st = lib().xm_ctx_create(device, ctypes.byref(self._h))
+        if st != XM_OK:
+            raise XmatchError(st, lib().xm_last_error_message(None).decode())
+
+    def _check(self, st):
+        if st != XM_OK:
+            raise XmatchError(st, lib().xm_last_error_message(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        """Destroy the context after every catalog and result made from it."""
+        if self._h:
+            for c in list(self._children):
+                c.close()
+            lib().xm_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_timing(self, on=True):
+        self._check(lib().xm_ctx_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        t = ctypes.c_double()
+        c = ctypes.c_uint64()
+        self._check(lib().xm_ctx_kernel_time(self._h, name.encode(), ctypes.byref(t), ctypes.byref(c)))
+        return t.value, c.value
+
+    def reset_timing(self):
+        self._check(lib().xm_ctx_reset_timing(self._h))
+
+    def synchronize(self):
+        self._check(lib().xm_ctx_synchronize(self._h))
+
+    def to_device(self, arr):
+        """Copy a host array to a device buffer owned by the context pool."""
+        arr = np.ascontiguousarray(arr)
+        d = ctypes.c_void_p()
+        self._check(lib().xm_device_alloc(self._h, max(arr.nbytes, 1), ctypes.byref(d)))
+        self._check(lib().xm_memcpy_h2d(self._h, d, _ptr(arr), arr.nbytes))
+        return DeviceBuffer(self, d, arr.nbytes, arr.dtype, arr.shape)
+
+    def catalog(self, records, zone_height_deg):
+        """Zone-sorted device catalog from host records (dtype OBJ_DTYPE) or a DeviceBuffer."""
+        return Catalog(self, records, zone_height_deg)
+
+    def match(self, a, b, radius_deg, precision=FP64_PLAIN, capacity_hint=0):
+        r = ctypes.c_void_p()
+        stats = MatchStats()
+        self._check(lib().xm_match(self._h, a.handle, b.handle, radius_deg, precision,
+                                   capacity_hint, ctypes.byref(r), ctypes.byref(stats)))
+        return Result(self, r, stats)
+
+    def work_counts(self, a, b, radius_deg):
+        w = [ctypes.c_uint64() for _ in range(3)]
+        self._check(lib().xm_work_counts(self._h, a.handle, b.handle, radius_deg,
+                                         *(ctypes.byref(x) for x in w)))
+        return {"w_ref": w[0].value, "w_cand": w[1].value, "w_tile": w[2].value}
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, ptr, nbytes, dtype, shape):
+        self.ctx, self.ptr, self.nbytes, self.dtype, self.shape = ctx, ptr, nbytes, dtype, shape
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(lib().xm_device_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Catalog:
+    def __init__(self, ctx, records, zone_height_deg):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        if isinstance(records, DeviceBuffer):
+            n = records.nbytes // OBJ_DTYPE.itemsize
+            st = lib().xm_catalog_create_device(ctx.handle, records.ptr, n, zone_height_deg,
+                                                ctypes.byref(self._h))
+        else:
+            rec = np.ascontiguousarray(records)
+            if rec.dtype != OBJ_DTYPE:
+                raise TypeError("records must have dtype %r" % (OBJ_DTYPE,))
+            st = lib().xm_catalog_create(ctx.handle, _ptr(rec), rec.size, zone_height_deg,
+                                         ctypes.byref(self._h))
+        ctx._check(st)
+        ctx._children.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n = ctypes.c_uint64()
+        h = ctypes.c_double()
+        z0 = ctypes.c_int32()
+        z1 = ctypes.c_int32()
+        self.ctx._check(lib().xm_catalog_info(self._h, ctypes.byref(n), ctypes.byref(h),
+                                              ctypes.byref(z0), ctypes.byref(z1)))
+        return {"n": n.value, "zone_height_deg": h.value, "zmin": z0.value, "zmax": z1.value}
+
+    def export(self):
+        n = self.info()["n"]
+        out = {"id": np.empty(n, np.int64), "x": np.empty(n), "y": np.empty(n),
+               "z": np.empty(n), "zone": np.empty(n, np.int32)}
+        self.ctx._check(lib().xm_catalog_export(self.ctx.handle, self._h, _ptr(out["id"]),
+                                                _ptr(out["x"]), _ptr(out["y"]), _ptr(out["z"]),
+                                                _ptr(out["zone"])))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().xm_catalog_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Result:
+    def __init__(self, ctx, h, stats):
+        self.ctx = ctx
+        self._h = h
+        self.stats = stats
+        ctx._children.add(self)
+
+    @property
+    def count(self):
+        n = ctypes.c_uint64()
+        self.ctx._check(lib().xm_result_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def to_host(self, sort=True):
+        n = self.count
+        ida = np.empty(n, np.int64)
+        idb = np.empty(n, np.int64)
+        dist = np.empty(n, np.float64)
+        self.ctx._check(lib().xm_result_copy(self.ctx.handle, self._h, _ptr(ida), _ptr(idb),
+                                             _ptr(dist)))
+        if sort:
+            o = np.lexsort((idb, ida))
+            ida, idb, dist = ida[o], idb[o], dist[o]
+        return ida, idb, dist
+
+    def close(self):
+        if self._h:
+            lib().xm_result_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,205 @@
+// xm_catalog.hip -- segment preparation kernels: keys, gather to SoA unit
+// vectors, zone offsets, A-row chunks.
+//
+// Replaces Sorter::Sort (BoostXmatch/BoostXmatch/Sorter.cu:141-185):
+//   GetZoneId + less_zonera (Sorter.cu:20-43)   -> k_make_keys (packed sort keys)
+//   lower_bound/upper_bound (Sorter.cu:161-164) -> k_zone_offsets
+//   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (ids + FP64 unit vectors,
+//                                                  radec2xyz of Worker.cu:31-40 done once
+//                                                  per source instead of per pair)
+#include "xm_internal.h"
+
+namespace xm {
+
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ inline bool valid_record(double ra, double dec) {
+    return ra >= 0.0 && ra < 360.0 && dec >= -90.0 && dec <= 90.0;  // false for NaN
+}
+
+__global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ recs, uint32_t n,
+                                                   double h_deg, int32_t n_zones,
+                                                   uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals,
+                                                   int32_t* __restrict__ zminmax,
+                                                   uint32_t* __restrict__ bad) {
+    __shared__ int32_t s_min[TPB / WAVE], s_max[TPB / WAVE];
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    int32_t zmn = 0x7fffffff, zmx = -1;
+    uint32_t nbad = 0;
+    if (i < n) {
+        const double ra = recs[i].ra_deg;
+        const double dec = recs[i].dec_deg;
+        uint64_t key = ~uint64_t(0);
+        if (valid_record(ra, dec)) {
+            const int32_t z = zone_of(dec, h_deg, n_zones);
+            key = make_key(z, ra);
+            zmn = z;
+            zmx = z;
+        } else {
+            nbad = 1;
+        }
+        keys[i] = key;
+        vals[i] = i;
+    }
+    // wave then block reduction, one atomic per block
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        zmn = min(zmn, __shfl_xor(zmn, d, WAVE));
+        zmx = max(zmx, __shfl_xor(zmx, d, WAVE));
+        nbad += __shfl_xor(nbad, d, WAVE);
+    }
+    const int w = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        s_min[w] = zmn;
+        s_max[w] = zmx;
+        if (nbad) atomicAdd(bad, nbad);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t a = s_min[0], b = s_max[0];
+        for (int q = 1; q < TPB / WAVE; q++) {
+            a = min(a, s_min[q]);
+            b = max(b, s_max[q]);
+        }
+        if (b >= 0) {
+            atomicMin(&zminmax[0], a);
+            atomicMax(&zminmax[1], b);
+        }
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
+                                                const uint32_t* __restrict__ perm, uint32_t n,
+                                                int64_t* __restrict__ id, double* __restrict__ x,
+                                                double* __restrict__ y, double* __restrict__ z) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    const int64_t oid = recs[s].id;
+    const double ra = recs[s].ra_deg;
+    const double dec = recs[s].dec_deg;
+    double ux, uy, uz;
+    unit_vector(ra, dec, &ux, &uy, &uz);
+    id[i] = oid;
+    x[i] = ux;
+    y[i] = uy;
+    z[i] = uz;
+}
+
+// zoff[z - zmin] = first sorted row whose zone is >= z, for z in [zmin, zmax+1].
+// Row i writes the entries of the zones in (zone(i-1), zone(i)].
+__global__ __launch_bounds__(TPB) void k_zone_offsets(const uint64_t* __restrict__ keys, uint32_t n,
+                                                      int32_t zmin, int32_t zmax,
+                                                      uint32_t* __restrict__ zoff) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i > n) return;
+    const int32_t zp = i == 0 ? zmin - 1 : key_zone(keys[i - 1]);
+    const int32_t zc = i == n ? zmax + 1 : key_zone(keys[i]);
+    for (int32_t z = zp + 1; z <= zc; z++) zoff[z - zmin] = i;
+}
+
+__global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict__ zoff,
+                                                      int32_t nzone,
+                                                      uint32_t* __restrict__ counts) {
+    const int32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= nzone) return;
+    const uint32_t rows = zoff[k + 1] - zoff[k];
+    counts[k] = (rows + CHUNK_ROWS - 1) / CHUNK_ROWS;
+}
+
+__global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__ zoff,
+                                                    const uint32_t* __restrict__ cstart,
+                                                    int32_t zmin, int32_t nzone,
+                                                    Chunk* __restrict__ chunks) {
+    const int32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= nzone) return;
+    const uint32_t r0 = zoff[k], r1 = zoff[k + 1];
+    uint32_t c = cstart[k];
+    for (uint32_t r = r0; r < r1; r += CHUNK_ROWS, c++) {
+        Chunk ch;
+        ch.start = r;
+        ch.count = (r1 - r) < uint32_t(CHUNK_ROWS) ? (r1 - r) : uint32_t(CHUNK_ROWS);
+        ch.zone = zmin + k;
+        ch.pad = 0;
+        chunks[c] = ch;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_fill(T* __restrict__ p, uint64_t n, T v) {
+    const uint64_t i = uint64_t(blockIdx.x) * TPB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
+
+}  // namespace
+
+void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
+                      uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad) {
+    if (n == 0) return;
+    launch_timed(ctx, "make_keys", [&] {
+        hipLaunchKernelGGL(k_make_keys, grid_for(n), dim3(TPB), 0, ctx->stream, recs, uint32_t(n),
+                           h_deg, n_zones, keys, vals, zminmax, bad);
+    });
+}
+
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, int64_t* id,
+                   double* x, double* y, double* z) {
+    if (n == 0) return;
+    launch_timed(ctx, "gather", [&] {
+        hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, perm,
+                           uint32_t(n), id, x, y, z);
+    });
+}
+
+void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin, int32_t zmax,
+                         uint32_t* zoff) {
+    launch_timed(ctx, "zone_offsets", [&] {
+        hipLaunchKernelGGL(k_zone_offsets, grid_for(n + 1), dim3(TPB), 0, ctx->stream, keys,
+                           uint32_t(n), zmin, zmax, zoff);
+    });
+}
+
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts) {
+    if (nzone <= 0) return;
+    launch_timed(ctx, "chunk_counts", [&] {
+        hipLaunchKernelGGL(k_chunk_counts, grid_for(uint64_t(nzone)), dim3(TPB), 0, ctx->stream,
+                           zoff, nzone, counts);
+    });
+}
+
+void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart, int32_t zmin,
+                       int32_t nzone, Chunk* chunks) {
+    if (nzone <= 0) return;
+    launch_timed(ctx, "chunk_fill", [&] {
+        hipLaunchKernelGGL(k_chunk_fill, grid_for(uint64_t(nzone)), dim3(TPB), 0, ctx->stream,
+                           zoff, cstart, zmin, nzone, chunks);
+    });
+}
+
+void launch_fill_u32(xm_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v) {
+    if (n == 0) return;
+    launch_timed(ctx, "fill", [&] {
+        hipLaunchKernelGGL(k_fill<uint32_t>, grid_for(n), dim3(TPB), 0, ctx->stream, p, n, v);
+    });
+}
+
+void launch_fill_u64(xm_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v) {
+    if (n == 0) return;
+    launch_timed(ctx, "fill", [&] {
+        hipLaunchKernelGGL(k_fill<uint64_t>, grid_for(n), dim3(TPB), 0, ctx->stream, p, n, v);
+    });
+}
+
+void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v) {
+    if (n == 0) return;
+    launch_timed(ctx, "fill", [&] {
+        hipLaunchKernelGGL(k_fill<int32_t>, grid_for(n), dim3(TPB), 0, ctx->stream, p, n, v);
+    });
+}
+
+}  // namespace xm

@@ -1,0 +1,148 @@
+// xm_internal.h -- library-internal types shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/xmatch.h"
+#include "xm_geom.h"
+
+namespace xm {
+
+// Result of a failed HIP call, carried up to the C ABI as XM_EDEVICE.
+struct DeviceError {
+    hipError_t code;
+    std::string what;
+};
+struct InvalidArg {
+    std::string what;
+};
+struct OutOfMemory {
+    std::string what;
+};
+
+#define XM_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::xm::DeviceError{e_, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// Per-launch timing records (HIP events on the context stream).
+struct TimedLaunch {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace xm
+
+struct xm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string last_error = "no error";
+    bool timing = false;
+    std::vector<xm::TimedLaunch> launches;   // pending, resolved on query
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
+    // pinned host scratch for small device->host reads
+    void* pinned = nullptr;
+};
+
+struct xm_catalog {
+    xm_ctx* ctx = nullptr;
+    uint64_t n = 0;
+    double h_deg = 0;
+    int32_t n_zones = 0;
+    int32_t zmin = 0, zmax = -1;
+    uint64_t* key = nullptr;     // sorted (zone << 40 | ra quantum)
+    int64_t* id = nullptr;
+    double* x = nullptr;         // FP64 unit vectors, SoA, sorted order
+    double* y = nullptr;
+    double* z = nullptr;
+    uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
+    // A-side row chunks (built on first use as the A catalog of a join)
+    void* chunks = nullptr;
+    uint64_t n_chunks = 0;
+};
+
+struct xm_result {
+    xm_ctx* ctx = nullptr;
+    uint64_t n = 0;
+    uint64_t cap = 0;
+    int64_t* id_a = nullptr;
+    int64_t* id_b = nullptr;
+    double* dist = nullptr;
+};
+
+namespace xm {
+
+constexpr int WAVE = 64;
+constexpr int CHUNK_ROWS = 64;   // A rows per wavefront in the pair kernel
+
+struct Chunk {
+    uint32_t start;
+    uint32_t count;
+    int32_t zone;
+    uint32_t pad;
+};
+
+// ---- memory (stream-ordered pool on the context stream)
+void* dalloc(xm_ctx* ctx, size_t bytes);
+void dfree(xm_ctx* ctx, void* p);
+
+// ---- launch bookkeeping: events around a kernel when timing is on
+void timing_begin(xm_ctx* ctx, const char* name, hipEvent_t* ev);
+void timing_end(xm_ctx* ctx, const char* name, hipEvent_t ev);
+
+template <typename F>
+inline void launch_timed(xm_ctx* ctx, const char* name, F&& f) {
+    hipEvent_t ev = nullptr;
+    if (ctx->timing) timing_begin(ctx, name, &ev);
+    f();
+    XM_HIP(hipGetLastError());
+    if (ctx->timing) timing_end(ctx, name, ev);
+}
+
+// ---- scan / sort (xm_sort.hip)
+// Exclusive prefix sum of n uint32 values in place.
+void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
+// LSD radix sort of (key, value) pairs on key bits [lo_bit, hi_bit); stable.
+// Buffers are ping-ponged; on return *keys / *vals point at the sorted data and
+// the other buffers hold scratch.
+void radix_sort_pairs(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint32_t** vals,
+                      uint32_t** vals_alt, uint64_t n, int lo_bit, int hi_bit,
+                      uint64_t skip_mask_digits);
+
+// ---- catalog kernels (xm_catalog.hip)
+void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
+                      uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad);
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n,
+                   int64_t* id, double* x, double* y, double* z);
+void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin,
+                         int32_t zmax, uint32_t* zoff);
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts);
+void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart, int32_t zmin,
+                       int32_t nzone, Chunk* chunks);
+void launch_fill_u32(xm_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
+void launch_fill_u64(xm_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v);
+void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
+
+// ---- pair kernel (xm_pair.hip)
+struct PairParams {
+    double theta_deg;       // search radius, degrees
+    double h_deg;           // zone height, degrees
+    double atan_sin_theta;  // atan(sin(theta)), host-computed once
+    double chord2;          // (2 sin(r/2))^2
+    int32_t nz;             // ceil(r / h)
+    int32_t n_zones;
+};
+void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
+                      const PairParams& pp, uint64_t* counter, uint64_t cap, int64_t* out_a,
+                      int64_t* out_b, double* out_d);
+
+}  // namespace xm

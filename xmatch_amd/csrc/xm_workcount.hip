@@ -1,0 +1,137 @@
+// xm_workcount.hip -- host-side work counts of a join, from the catalogs' sorted
+// keys and zone tables copied to the host (never from device counters).
+//
+//   w_ref  = sum_z nA(z) * sum_{|z'-z|<=nz} nB(z')
+//            the dense tests of the reference's zone loop (Worker.cu:330-359)
+//   w_cand = #{(a,b): |za-zb| <= nz, cyclic |ra_a-ra_b| <= alpha(za)}
+//            the pairs that reach the reference's chord test (Worker.cu:89-122),
+//            alpha = calc_alpha (Common.cu:16-23) without widening
+//   w_tile = sum over A chunks of rows(chunk) * B rows in the chunk's windows
+//            the tests the pair kernel executes (same integer window logic as
+//            the kernel: alpha_quanta + window_intervals)
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "xm_internal.h"
+
+using namespace xm;
+
+namespace {
+
+double ref_alpha_rad(double theta_deg, double h_deg, int32_t zone) {
+    const double d1 = std::fabs(zone * h_deg - 90.0);
+    const double d2 = std::fabs((zone + 1) * h_deg - 90.0);
+    const double d = std::max(d1, d2) + 0.01 * theta_deg;
+    if (d + theta_deg > 89.99) return -1.0;  // whole circle
+    return std::fabs(std::atan(std::sin(theta_deg / RAD2DEG)) /
+                     std::sqrt(std::fabs(std::cos((d - theta_deg) / RAD2DEG) *
+                                         std::cos((d + theta_deg) / RAD2DEG))));
+}
+
+uint64_t lower(const std::vector<uint64_t>& k, uint64_t lo, uint64_t hi, uint64_t t) {
+    return uint64_t(std::lower_bound(k.begin() + lo, k.begin() + hi, t) - k.begin());
+}
+
+}  // namespace
+
+extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b,
+                                    double radius_deg, uint64_t* w_ref, uint64_t* w_cand,
+                                    uint64_t* w_tile) {
+    if (!ctx || !a || !b) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        XM_HIP(hipSetDevice(ctx->device));
+        if (!(radius_deg > 0.0) || a->h_deg != b->h_deg) {
+            ctx->last_error = "bad radius or zone heights differ";
+            return XM_EINVAL;
+        }
+        std::vector<uint64_t> ka(a->n), kb(b->n);
+        const int64_t nta = int64_t(a->zmax) - a->zmin + 2, ntb = int64_t(b->zmax) - b->zmin + 2;
+        std::vector<uint32_t> za(size_t(std::max<int64_t>(nta, 1))), zb(size_t(std::max<int64_t>(ntb, 1)));
+        if (a->n) XM_HIP(hipMemcpyAsync(ka.data(), a->key, a->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (b->n) XM_HIP(hipMemcpyAsync(kb.data(), b->key, b->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        XM_HIP(hipMemcpyAsync(za.data(), a->zoff, nta * 4, hipMemcpyDeviceToHost, ctx->stream));
+        XM_HIP(hipMemcpyAsync(zb.data(), b->zoff, ntb * 4, hipMemcpyDeviceToHost, ctx->stream));
+        XM_HIP(hipStreamSynchronize(ctx->stream));
+
+        const double h = a->h_deg;
+        const int32_t nz = int32_t(std::ceil(radius_deg / h));
+        const int32_t n_zones = a->n_zones;
+        const double atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
+        const int64_t full = int64_t(1) << RA_BITS;
+
+        std::atomic<int64_t> next{a->n ? a->zmin : 1};
+        const int64_t zlast = a->n ? a->zmax : 0;
+        unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::vector<uint64_t> r_ref(nth, 0), r_cand(nth, 0), r_tile(nth, 0);
+        auto work = [&](unsigned t) {
+            uint64_t s_ref = 0, s_cand = 0, s_tile = 0;
+            for (;;) {
+                const int64_t z = next.fetch_add(1);
+                if (z > zlast) break;
+                const uint64_t a0 = za[z - a->zmin], a1 = za[z - a->zmin + 1];
+                if (a1 <= a0) continue;
+                const double al = ref_alpha_rad(radius_deg, h, int32_t(z));
+                const double al_q = al < 0 ? -1.0 : al * RAD2DEG * RA_SCALE;
+                const int64_t aq = alpha_quanta(radius_deg, h, int32_t(z), atan_sin_theta);
+                for (int64_t z2 = std::max<int64_t>(0, z - nz);
+                     z2 <= std::min<int64_t>(n_zones - 1, z + nz); z2++) {
+                    if (z2 < b->zmin || z2 > b->zmax) continue;
+                    const uint64_t b0 = zb[z2 - b->zmin], b1 = zb[z2 - b->zmin + 1];
+                    if (b1 <= b0) continue;
+                    s_ref += (a1 - a0) * (b1 - b0);
+                    const uint64_t zk = uint64_t(z2) << RA_BITS;
+                    // w_cand: per A row, B rows within the unwidened alpha (cyclic)
+                    for (uint64_t i = a0; i < a1; i++) {
+                        if (al_q < 0 || 2 * al_q + 1 >= double(full)) { s_cand += b1 - b0; continue; }
+                        const double q = double(ka[i] & RA_MASK);
+                        const double lo = q - al_q, hi = q + al_q;
+                        auto cnt = [&](double l, double r) -> uint64_t {  // keys with l <= q <= r
+                            const uint64_t li = uint64_t(std::max(0.0, std::ceil(l)));
+                            const double rr = std::floor(r);
+                            if (rr < 0 || double(li) > rr) return 0;
+                            const uint64_t ri = uint64_t(std::min(rr, double(RA_MASK)));
+                            return lower(kb, b0, b1, zk + ri + 1) - lower(kb, b0, b1, zk + li);
+                        };
+                        if (lo < 0) s_cand += cnt(lo + full, double(RA_MASK)) + cnt(0, hi);
+                        else if (hi > double(RA_MASK)) s_cand += cnt(lo, double(RA_MASK)) + cnt(0, hi - full);
+                        else s_cand += cnt(lo, hi);
+                    }
+                    // w_tile: the kernel's chunks and windows
+                    for (uint64_t s = a0; s < a1; s += CHUNK_ROWS) {
+                        const uint64_t cnt = std::min<uint64_t>(CHUNK_ROWS, a1 - s);
+                        const uint64_t q_lo = ka[s] & RA_MASK, q_hi = ka[s + cnt - 1] & RA_MASK;
+                        uint64_t lo[2], hi[2];
+                        const int ni = window_intervals(int32_t(z2), q_lo, q_hi, aq, lo, hi);
+                        for (int t2 = 0; t2 < ni; t2++) {
+                            const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
+                            const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                            s_tile += cnt * (j1 - j0);
+                        }
+                    }
+                }
+            }
+            r_ref[t] = s_ref;
+            r_cand[t] = s_cand;
+            r_tile[t] = s_tile;
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth; t++) th.emplace_back(work, t);
+        for (auto& t : th) t.join();
+        uint64_t s1 = 0, s2 = 0, s3 = 0;
+        for (unsigned t = 0; t < nth; t++) { s1 += r_ref[t]; s2 += r_cand[t]; s3 += r_tile[t]; }
+        if (w_ref) *w_ref = s1;
+        if (w_cand) *w_cand = s2;
+        if (w_tile) *w_tile = s3;
+        return XM_OK;
+    } catch (const DeviceError& e) {
+        ctx->last_error = e.what;
+        return XM_EDEVICE;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return XM_EINTERNAL;
+    }
+}
