@@ -28,7 +28,8 @@
  * Boundary rule: the dec box and the alpha window are pruning only.  They are
  * widened by a margin (relative 1e-6 plus 1e-12 rad) that exceeds every rounding
  * error of the chord, so the match set is exactly
- *   { (a,b) : |zone_a - zone_b| <= nz  and  d2(u_a,u_b) < sr_dist2 }.
+ *   { (a,b) : |zone_a - zone_b| <= nz  and  d2(u_a,u_b) < sr_dist2 },
+ * d2 = fma(dz, dz, fma(dy, dy, dx*dx)) on the FP64 unit-vector differences.
  */
 #ifndef XMO_H
 #define XMO_H

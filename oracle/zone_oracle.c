@@ -3,9 +3,11 @@
  * path).  FP64 restatement of the BoostXmatch zone join; see xmo.h for the map of
  * reference functions and the corrected deviations.
  *
- * Build: oracle/Makefile (gcc -O3 -ffp-contract=off).  Contraction must stay off:
- * d2 = dx*dx + dy*dy + dz*dz is evaluated as three roundings of products and two
- * of sums, in that order, exactly as the device kernel does.
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off).  The chord is evaluated as
+ *   d2 = fma(dz, dz, fma(dy, dy, dx*dx))
+ * with the exactly rounded C99 fma(), the same operations the device kernel issues
+ * (and the contraction nvcc applies by default to the reference's
+ * dist2 = x*x + y*y + z*z, Worker.cu:96).  Nothing else may be contracted.
  */
 #define _GNU_SOURCE
 #include "xmo.h"
@@ -305,7 +307,7 @@ static int xmo_test_pair(xmo_worker *w, int64_t ia, int64_t ib)
     const double dx = A->x[ia] - B->x[ib];
     const double dy = A->y[ia] - B->y[ib];
     const double dz = A->z[ia] - B->z[ib];
-    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if (d2 < w->job->sr_dist2)
         return xmo_pairs_push(&w->out, A->id[ia], B->id[ib], xmo_dist_arcsec(d2));
     return 0;
@@ -454,7 +456,7 @@ int xmo_brute(const xmo_obj *a, int64_t na, const xmo_obj *b, int64_t nb, double
             const int32_t dzn = za - bz[j];
             if (dzn > nz || -dzn > nz) continue;
             const double dx = x - bx[3 * j], dy = y - bx[3 * j + 1], dz = z - bx[3 * j + 2];
-            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
             if (d2 < sr_dist2) {
                 err = xmo_pairs_push(out, a[i].id, b[j].id, xmo_dist_arcsec(d2));
                 if (err) break;

@@ -10,7 +10,7 @@
 // (SGPR) operand and every lane tests its A row against it.
 //
 // Predicate (Worker.cu:88-97 with the seam bug D2 corrected):
-//   d2 = dx*dx + dy*dy + dz*dz  (no contraction)  <  (2 sin(r/2))^2
+//   d2 = fma(dz, dz, fma(dy, dy, dx*dx))  <  (2 sin(r/2))^2
 // Hits are compacted with ballot + mbcnt into a per-wave LDS stage; a full stage
 // (or the end of the chunk) costs one global atomic on the 64-bit append counter.
 // Slots beyond the buffer capacity are counted but not written; the host then
@@ -25,6 +25,7 @@ namespace {
 constexpr int PK_WAVES = 4;
 constexpr int PK_THREADS = PK_WAVES * WAVE;
 constexpr int STAGE = 256;
+constexpr int UNROLL = 4;
 
 __device__ inline uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -87,6 +88,43 @@ __device__ inline void flush_stage(uint32_t nst, uint32_t lane, const uint32_t* 
     __builtin_amdgcn_wave_barrier();
 }
 
+// The chord of the reference's exact test (Worker.cu:93-97) on FP64 unit vectors:
+// d2 = fma(dz, dz, fma(dy, dy, dx*dx)), identical on host and device.
+__device__ inline double chord2(double ax, double ay, double az, double bx, double by, double bz) {
+    const double dx = ax - bx;
+    const double dy = ay - by;
+    const double dz = az - bz;
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+
+// Append the lanes of hit mask m (A row per lane, B row j) to the wave's stage;
+// flush it to the result buffer when it cannot take another full wave of hits.
+__device__ inline void stage_hits(uint64_t m, double d2, uint32_t row, uint32_t j, uint32_t lane,
+                                  uint32_t& nst, uint32_t* st_ra, uint32_t* st_rb, double* st_d2,
+                                  const int64_t* __restrict__ aid, const int64_t* __restrict__ bid,
+                                  unsigned long long* __restrict__ counter, uint64_t cap,
+                                  int64_t* __restrict__ out_a, int64_t* __restrict__ out_b,
+                                  double* __restrict__ out_d) {
+    if ((m >> lane) & 1u) {
+        const uint32_t pos = nst + mbcnt64(m);
+        st_ra[pos] = row;
+        st_rb[pos] = j;
+        st_d2[pos] = d2;
+    }
+    nst += uint32_t(__popcll(m));
+    if (nst > uint32_t(STAGE - WAVE)) {
+        flush_stage(nst, lane, st_ra, st_rb, st_d2, aid, bid, counter, cap, out_a, out_b, out_d);
+        nst = 0;
+    }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of
+// chunks so that neighbouring chunks, which read the same B zones, share an L2.
+__device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
     const Chunk* __restrict__ chunks, uint32_t n_chunks, const uint64_t* __restrict__ akey,
     const double* __restrict__ ax_, const double* __restrict__ ay_, const double* __restrict__ az_,
@@ -101,7 +139,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
 
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
-    const uint32_t c = blockIdx.x * PK_WAVES + w;
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
     if (c >= n_chunks) return;  // whole wavefront; the kernel has no block barrier
 
     const uint32_t start = uni(chunks[c].start);
@@ -136,27 +174,34 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
         for (int t = 0; t < ni; t++) {
             const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
             const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
-            for (uint32_t j = j0; j < j1; j++) {
-                const double dx = ax - bx_[j];
-                const double dy = ay - by_[j];
-                const double dzv = az - bz_[j];
-                const double d2 = dx * dx + dy * dy + dzv * dzv;
-                const bool hit = d2 < pp.chord2;
-                const uint64_t m = __ballot(hit);
-                if (m) {
-                    if (hit) {
-                        const uint32_t pos = nst + mbcnt64(m);
-                        st_ra[pos] = row;
-                        st_rb[pos] = j;
-                        st_d2[pos] = d2;
-                    }
-                    nst += uint32_t(__popcll(m));
-                    if (nst > uint32_t(STAGE - WAVE)) {
-                        flush_stage(nst, lane, st_ra, st_rb, st_d2, aid, bid, counter, cap, out_a,
-                                    out_b, out_d);
-                        nst = 0;
-                    }
+            uint32_t j = j0;
+            // UNROLL B rows per step: each coordinate arrives as one wide scalar load
+            // from a uniform base, and one branch covers the UNROLL hit masks
+            for (; j + UNROLL <= j1; j += UNROLL) {
+                const double* __restrict__ px = bx_ + size_t(j);
+                const double* __restrict__ py = by_ + size_t(j);
+                const double* __restrict__ pz = bz_ + size_t(j);
+                double d2[UNROLL];
+                uint64_t m[UNROLL];
+                uint64_t any = 0;
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    d2[u] = chord2(ax, ay, az, px[u], py[u], pz[u]);
+                    m[u] = __ballot(d2[u] < pp.chord2);
+                    any |= m[u];
                 }
+                if (any) {
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++)
+                        if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, nst, st_ra, st_rb,
+                                             st_d2, aid, bid, counter, cap, out_a, out_b, out_d);
+                }
+            }
+            for (; j < j1; j++) {
+                const double d2 = chord2(ax, ay, az, bx_[j], by_[j], bz_[j]);
+                const uint64_t m = __ballot(d2 < pp.chord2);
+                if (m) stage_hits(m, d2, row, j, lane, nst, st_ra, st_rb, st_d2, aid, bid, counter,
+                                  cap, out_a, out_b, out_d);
             }
         }
     }
