@@ -23,10 +23,13 @@
 namespace xm {
 
 constexpr double RAD2DEG = 57.295779513082323;  // Common.h:9
-constexpr int RA_BITS = 40;                      // ra quantisation inside a sort key
+// Sort key = zone << RA_BITS | ra quantum, ra quantised to 2^-24 turn (0.077 arcsec).
+// Every key bit is sorted, so keys are totally ordered and searchable; rows inside one
+// quantum keep their input order (the reference leaves (zone, ra) ties unordered).
+constexpr int RA_BITS = 24;
 constexpr uint64_t RA_MASK = (uint64_t(1) << RA_BITS) - 1;
 constexpr double RA_SCALE = double(uint64_t(1) << RA_BITS) / 360.0;  // quanta per degree
-constexpr int SORT_LO_BIT = 16;  // keys are ordered on bits [16, 64): ra to 2^-24 turn
+constexpr int SORT_LO_BIT = 0;
 constexpr int64_t WHOLE_ZONE = -1;
 
 // ---------------------------------------------------------------------------
