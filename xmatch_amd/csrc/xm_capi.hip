@@ -222,8 +222,9 @@ void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, 
         const int hi_bit = RA_BITS + bit_length(uint32_t(zmax));
         uint64_t skip = 0;
         for (int d = RA_BITS / 8; d < 8; d++) {
-            const int zs = d * 8 - RA_BITS;
-            if ((uint32_t(zmin) >> zs) == (uint32_t(zmax) >> zs)) skip |= uint64_t(1) << d;
+            const int zs = d * 8 - RA_BITS;  // first zone bit inside digit d (0..40)
+            if ((uint64_t(uint32_t(zmin)) >> zs) == (uint64_t(uint32_t(zmax)) >> zs))
+                skip |= uint64_t(1) << d;
         }
         radix_sort_pairs(ctx, &keys, &keys_alt, &vals, &vals_alt, n, SORT_LO_BIT, hi_bit, skip);
         dfree(ctx, keys_alt);

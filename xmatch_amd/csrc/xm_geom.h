@@ -30,6 +30,7 @@ constexpr int RA_BITS = 24;
 constexpr uint64_t RA_MASK = (uint64_t(1) << RA_BITS) - 1;
 constexpr double RA_SCALE = double(uint64_t(1) << RA_BITS) / 360.0;  // quanta per degree
 constexpr int SORT_LO_BIT = 0;
+static_assert(RA_BITS % 8 == 0 && RA_BITS <= 32, "zone digits start on a byte boundary");
 constexpr int64_t WHOLE_ZONE = -1;
 
 // ---------------------------------------------------------------------------
