@@ -72,6 +72,15 @@ xm_status xm_ctx_set_timing(xm_ctx *ctx, int enable);
 /* Sum and count of recorded durations for kernel `name` since the last reset. */
 xm_status xm_ctx_kernel_time(xm_ctx *ctx, const char *name, double *total_ms, uint64_t *count);
 xm_status xm_ctx_reset_timing(xm_ctx *ctx);
+/* Tuning knobs of the context (no counterpart in the reference, whose kernel
+ * shape was fixed at compile time, Worker.cu:22,351-353):
+ *   "dense_min"  a 64-row A chunk is joined densely (every B row of its window
+ *                against every chunk row) when a B row is expected to meet at
+ *                least this many chunk rows, else sparsely (each B row against the
+ *                chunk rows within the ra window).  0 = always dense.  Default 16.
+ * Results never depend on the knobs; only the work done does. */
+xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
+xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
 /* Wait for all device work of the context. */
 xm_status xm_ctx_synchronize(xm_ctx *ctx);
 /* Device malloc/free through the context's stream-ordered pool (for callers that

@@ -238,3 +238,25 @@ def test_full_sky_2m_vs_oracle(gpu_ctx):
     # the host count resolves ra to the key quantum (2^-24 turn), the oracle exactly
     assert abs(w["w_cand"] - cnt["w_cand"]) <= 1e-3 * cnt["w_cand"] + 10
     assert w["w_tile"] >= w["w_cand"] * 0.99
+
+
+@pytest.mark.parametrize("dense_min", [0, 4, 16, 1000000])
+def test_join_mode_does_not_change_the_set(gpu_ctx, dense_min):
+    # dense_min 0 forces the dense tile loop on every chunk, 1e6 the sparse window
+    # search on every chunk the pole rule allows; the set must not move
+    A, B, r = synth.config_full_sky(seed=9, n_a=400_000, n_b=400_000)
+    clus = synth.make_objs(np.arange(5_000_000, 5_003_000),
+                           *synth.uniform_patch(np.random.default_rng(1), 3000, 150, 150.01, 20,
+                                                20.001))
+    A = np.concatenate([A, clus])
+    B = np.concatenate([B, clus])
+    B["id"][-3000:] += 10**9
+    ref, _ = O.match(A, B, r)
+    old = gpu_ctx.get_param("dense_min")
+    try:
+        gpu_ctx.set_param("dense_min", dense_min)
+        assert gpu_ctx.get_param("dense_min") == dense_min
+        got, _ = gpu_join(gpu_ctx, A, B, r)
+    finally:
+        gpu_ctx.set_param("dense_min", old)
+    assert_same(got, ref)

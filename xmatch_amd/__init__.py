@@ -31,6 +31,7 @@ _STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_E
 EXPORTED = [
     "xm_abi_version", "xm_ctx_create", "xm_ctx_destroy", "xm_last_error_message",
     "xm_ctx_set_timing", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
+    "xm_ctx_set_param", "xm_ctx_get_param",
     "xm_device_alloc", "xm_device_free", "xm_memcpy_h2d", "xm_memcpy_d2h",
     "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
     "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
@@ -76,6 +77,9 @@ def lib():
                                     ctypes.POINTER(u64)], ctypes.c_int),
             "xm_ctx_reset_timing": ([vp], ctypes.c_int),
             "xm_ctx_synchronize": ([vp], ctypes.c_int),
+            "xm_ctx_set_param": ([vp, ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
+            "xm_ctx_get_param": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
+                                 ctypes.c_int),
             "xm_device_alloc": ([vp, ctypes.c_size_t, pvp], ctypes.c_int),
             "xm_device_free": ([vp, vp], ctypes.c_int),
             "xm_memcpy_h2d": ([vp, vp, vp, ctypes.c_size_t], ctypes.c_int),
@@ -153,6 +157,14 @@ class Context:
 
     def synchronize(self):
         self._check(lib().xm_ctx_synchronize(self._h))
+
+    def set_param(self, name, value):
+        self._check(lib().xm_ctx_set_param(self._h, name.encode(), int(value)))
+
+    def get_param(self, name):
+        v = ctypes.c_int64()
+        self._check(lib().xm_ctx_get_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def to_device(self, arr):
         """Copy a host array to a device buffer owned by the context pool."""

@@ -79,7 +79,7 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     A->n_chunks = total;
 }
 
-PairParams make_params(const xm_catalog* A, double radius_deg) {
+PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, double radius_deg) {
     PairParams pp;
     pp.theta_deg = radius_deg;
     pp.h_deg = A->h_deg;
@@ -87,6 +87,7 @@ PairParams make_params(const xm_catalog* A, double radius_deg) {
     pp.chord2 = chord2_of_radius(radius_deg);
     pp.nz = int32_t(std::ceil(radius_deg / A->h_deg));  // Worker.cu:313
     pp.n_zones = A->n_zones;
+    pp.dense_min = ctx->dense_min;
     return pp;
 }
 
@@ -324,6 +325,30 @@ xm_status xm_ctx_reset_timing(xm_ctx* ctx) {
     });
 }
 
+xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
+    if (!ctx || !name) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        const std::string n = name;
+        if (n == "dense_min") {
+            if (value < 0 || value > 1000000) throw InvalidArg{"dense_min out of range"};
+            ctx->dense_min = uint32_t(value);
+        } else {
+            throw InvalidArg{"unknown parameter '" + n + "'"};
+        }
+    });
+}
+
+xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
+    if (!ctx || !name || !value) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        const std::string n = name;
+        if (n == "dense_min") *value = ctx->dense_min;
+        else throw InvalidArg{"unknown parameter '" + n + "'"};
+    });
+}
+
 xm_status xm_ctx_synchronize(xm_ctx* ctx) {
     if (!ctx) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -473,7 +498,7 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             throw InvalidArg{"precision mode not available in this build"};
         xm_catalog* A = const_cast<xm_catalog*>(a);
         ensure_chunks(ctx, A);
-        const PairParams pp = make_params(A, radius_deg);
+        const PairParams pp = make_params(ctx, A, radius_deg);
         uint64_t cap = capacity_hint ? capacity_hint : 2 * std::max(a->n, b->n);
         cap = std::max<uint64_t>(cap, 1024);
         counter = static_cast<uint64_t*>(dalloc(ctx, sizeof(uint64_t)));

@@ -7,6 +7,8 @@
 //   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (ids + FP64 unit vectors,
 //                                                  radec2xyz of Worker.cu:31-40 done once
 //                                                  per source instead of per pair)
+#include <algorithm>
+
 #include "xm_internal.h"
 
 namespace xm {
@@ -19,6 +21,8 @@ __device__ inline bool valid_record(double ra, double dec) {
     return ra >= 0.0 && ra < 360.0 && dec >= -90.0 && dec <= 90.0;  // false for NaN
 }
 
+// Grid-stride over the records with a capped grid, so the zone range and the
+// bad-record count cost one atomic per block rather than one per 256 records.
 __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ recs, uint32_t n,
                                                    double h_deg, int32_t n_zones,
                                                    uint64_t* __restrict__ keys,
@@ -26,25 +30,25 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
                                                    int32_t* __restrict__ zminmax,
                                                    uint32_t* __restrict__ bad) {
     __shared__ int32_t s_min[TPB / WAVE], s_max[TPB / WAVE];
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    __shared__ uint32_t s_bad[TPB / WAVE];
     int32_t zmn = 0x7fffffff, zmx = -1;
     uint32_t nbad = 0;
-    if (i < n) {
+    const uint64_t stride = uint64_t(gridDim.x) * TPB;
+    for (uint64_t i = uint64_t(blockIdx.x) * TPB + threadIdx.x; i < n; i += stride) {
         const double ra = recs[i].ra_deg;
         const double dec = recs[i].dec_deg;
         uint64_t key = ~uint64_t(0);
         if (valid_record(ra, dec)) {
             const int32_t z = zone_of(dec, h_deg, n_zones);
             key = make_key(z, ra);
-            zmn = z;
-            zmx = z;
+            zmn = min(zmn, z);
+            zmx = max(zmx, z);
         } else {
-            nbad = 1;
+            nbad++;
         }
         keys[i] = key;
-        vals[i] = i;
+        vals[i] = uint32_t(i);
     }
-    // wave then block reduction, one atomic per block
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) {
         zmn = min(zmn, __shfl_xor(zmn, d, WAVE));
@@ -55,19 +59,22 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
     if ((threadIdx.x & (WAVE - 1)) == 0) {
         s_min[w] = zmn;
         s_max[w] = zmx;
-        if (nbad) atomicAdd(bad, nbad);
+        s_bad[w] = nbad;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int32_t a = s_min[0], b = s_max[0];
+        uint32_t nb = s_bad[0];
         for (int q = 1; q < TPB / WAVE; q++) {
             a = min(a, s_min[q]);
             b = max(b, s_max[q]);
+            nb += s_bad[q];
         }
         if (b >= 0) {
             atomicMin(&zminmax[0], a);
             atomicMax(&zminmax[1], b);
         }
+        if (nb) atomicAdd(bad, nb);
     }
 }
 
@@ -141,8 +148,9 @@ inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad) {
     if (n == 0) return;
+    const uint32_t nblk = std::min<uint32_t>(grid_for(n).x, 4096u);
     launch_timed(ctx, "make_keys", [&] {
-        hipLaunchKernelGGL(k_make_keys, grid_for(n), dim3(TPB), 0, ctx->stream, recs, uint32_t(n),
+        hipLaunchKernelGGL(k_make_keys, dim3(nblk), dim3(TPB), 0, ctx->stream, recs, uint32_t(n),
                            h_deg, n_zones, keys, vals, zminmax, bad);
     });
 }

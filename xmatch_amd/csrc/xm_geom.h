@@ -152,11 +152,14 @@ XM_HD int64_t alpha_quanta(double theta_deg, double h_deg, int32_t zone, double 
 
 // The (at most two) key intervals of B zone zb that a chunk whose A rows span ra
 // quanta [q_lo, q_hi] must test, for half-width aq (or WHOLE_ZONE).  Intervals are
-// inclusive key bounds [lo[i], hi[i]]; returns their count.
+// inclusive key bounds [lo[i], hi[i]]; shift[i] maps the ra quanta of that interval
+// into the chunk's frame across the 0/360 seam (-2^24, 0 or +2^24).  Returns the
+// interval count.
 XM_HD int window_intervals(int32_t zb, uint64_t q_lo, uint64_t q_hi, int64_t aq, uint64_t lo[2],
-                           uint64_t hi[2]) {
+                           uint64_t hi[2], int32_t shift[2]) {
     const uint64_t zk = uint64_t(uint32_t(zb)) << RA_BITS;
     const int64_t full = int64_t(uint64_t(1) << RA_BITS);
+    shift[0] = shift[1] = 0;
     if (aq == WHOLE_ZONE) {
         lo[0] = zk; hi[0] = zk | RA_MASK;
         return 1;
@@ -168,17 +171,32 @@ XM_HD int window_intervals(int32_t zb, uint64_t q_lo, uint64_t q_hi, int64_t aq,
         return 1;
     }
     if (l < 0) {
-        lo[0] = zk | uint64_t(l + full); hi[0] = zk | RA_MASK;
+        lo[0] = zk | uint64_t(l + full); hi[0] = zk | RA_MASK; shift[0] = -int32_t(full);
         lo[1] = zk; hi[1] = zk | uint64_t(h);
         return 2;
     }
     if (h >= full) {
         lo[0] = zk | uint64_t(l); hi[0] = zk | RA_MASK;
-        lo[1] = zk; hi[1] = zk | uint64_t(h - full);
+        lo[1] = zk; hi[1] = zk | uint64_t(h - full); shift[1] = int32_t(full);
         return 2;
     }
     lo[0] = zk | uint64_t(l); hi[0] = zk | uint64_t(h);
     return 1;
+}
+
+// Join mode of one A chunk.  Dense: every B row of the window is tested against
+// every row of the chunk (one wave instruction per B row).  Sparse: each B row is
+// tested only against the chunk rows within +-aq of its ra (binary search on the
+// chunk's ra quanta).  Dense is used for the pole rule, for windows that cover the
+// circle, and when a B row is expected to meet at least dense_min chunk rows:
+// 64 (2aq+1) / (span + 2aq + 1) >= dense_min.
+XM_HD bool chunk_is_dense(uint64_t q_lo, uint64_t q_hi, int64_t aq, uint32_t dense_min) {
+    if (aq == WHOLE_ZONE) return true;
+    const int64_t full = int64_t(uint64_t(1) << RA_BITS);
+    const int64_t w = 2 * aq + 1;
+    const int64_t span = int64_t(q_hi) - int64_t(q_lo);
+    if (span + w >= full) return true;
+    return 64 * w >= int64_t(dense_min) * (span + w);
 }
 
 }  // namespace xm

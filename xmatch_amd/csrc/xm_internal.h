@@ -51,6 +51,8 @@ struct xm_ctx {
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
     // pinned host scratch for small device->host reads
     void* pinned = nullptr;
+    // tuning knobs (xm_ctx_set_param)
+    uint32_t dense_min = 16;
 };
 
 struct xm_catalog {
@@ -59,7 +61,7 @@ struct xm_catalog {
     double h_deg = 0;
     int32_t n_zones = 0;
     int32_t zmin = 0, zmax = -1;
-    uint64_t* key = nullptr;     // sorted (zone << 40 | ra quantum)
+    uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
     int64_t* id = nullptr;
     double* x = nullptr;         // FP64 unit vectors, SoA, sorted order
     double* y = nullptr;
@@ -140,6 +142,7 @@ struct PairParams {
     double chord2;          // (2 sin(r/2))^2
     int32_t nz;             // ceil(r / h)
     int32_t n_zones;
+    uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
 };
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
                       const PairParams& pp, uint64_t* counter, uint64_t cap, int64_t* out_a,

@@ -100,16 +100,30 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
                         else if (hi > double(RA_MASK)) s_cand += cnt(lo, double(RA_MASK)) + cnt(0, hi - full);
                         else s_cand += cnt(lo, hi);
                     }
-                    // w_tile: the kernel's chunks and windows
+                    // w_tile: the kernel's chunks, windows and per-chunk mode
                     for (uint64_t s = a0; s < a1; s += CHUNK_ROWS) {
                         const uint64_t cnt = std::min<uint64_t>(CHUNK_ROWS, a1 - s);
                         const uint64_t q_lo = ka[s] & RA_MASK, q_hi = ka[s + cnt - 1] & RA_MASK;
+                        const bool dense = chunk_is_dense(q_lo, q_hi, aq, ctx->dense_min);
                         uint64_t lo[2], hi[2];
-                        const int ni = window_intervals(int32_t(z2), q_lo, q_hi, aq, lo, hi);
+                        int32_t shift[2];
+                        const int ni = window_intervals(int32_t(z2), q_lo, q_hi, aq, lo, hi, shift);
                         for (int t2 = 0; t2 < ni; t2++) {
                             const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
                             const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
-                            s_tile += cnt * (j1 - j0);
+                            if (dense) {
+                                s_tile += cnt * (j1 - j0);
+                                continue;
+                            }
+                            // sparse: chunk rows within +-aq of each B row's quantum
+                            int32_t qs[CHUNK_ROWS];
+                            for (uint64_t k = 0; k < cnt; k++) qs[k] = int32_t(ka[s + k] & RA_MASK);
+                            for (uint64_t jj = j0; jj < j1; jj++) {
+                                const int32_t qb = int32_t(kb[jj] & RA_MASK) + shift[t2];
+                                const int32_t* e0 = std::lower_bound(qs, qs + cnt, qb - int32_t(aq));
+                                const int32_t* e1 = std::upper_bound(qs, qs + cnt, qb + int32_t(aq));
+                                s_tile += uint64_t(e1 - e0);
+                            }
                         }
                     }
                 }
