@@ -40,7 +40,7 @@ BYTES_PER_SOURCE = {
 }
 KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
            "scan_apply", "gather", "zone_offsets", "chunk_counts", "chunk_fill", "fill",
-           "pair_fp64"]
+           "classify", "pair_sparse", "pair_dense", "finalize_dist"]
 
 
 def log(msg):
@@ -136,7 +136,7 @@ def main():
 
     # dominant kernel by total device time
     dom = max(ktimes.items(), key=lambda kv: kv[1]["total_ms"])[0]
-    if dom == "pair_fp64":
+    if dom in ("pair_sparse", "pair_dense"):
         # 9 FP64 VALU operations per executed pair test (3 sub, 3 mul, 2 add, 1 cmp)
         flops = w["w_tile"] * 9.0
         ach = flops / (ktimes[dom]["avg_ms"] / 1e3) / 1e12

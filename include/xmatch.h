@@ -59,6 +59,7 @@ typedef struct {
     uint64_t n_retries;      /* result-buffer overflows that forced a re-run */
     uint64_t band_hits;      /* FP32 filter only: pairs inside the guard band (FP64 rechecked) */
     double join_ms;          /* device time of the join (HIP events), including re-runs */
+    uint64_t n_dense_chunks; /* chunks joined in dense mode (the rest sparse) */
 } xm_match_stats;
 
 /* ---- context: replaces CudaManager/CudaContext (CudaManager.cpp:8-65,

@@ -176,14 +176,10 @@ void catalog_free(xm_catalog* c) {
     if (!c || !c->ctx) return;
     xm_ctx* ctx = c->ctx;
     dfree(ctx, c->key);
-    dfree(ctx, c->id);
-    dfree(ctx, c->x);
-    dfree(ctx, c->y);
-    dfree(ctx, c->z);
+    dfree(ctx, c->row);
     dfree(ctx, c->zoff);
     dfree(ctx, c->chunks);
-    c->key = nullptr; c->id = nullptr; c->x = c->y = c->z = nullptr;
-    c->zoff = nullptr; c->chunks = nullptr;
+    c->key = nullptr; c->row = nullptr; c->zoff = nullptr; c->chunks = nullptr;
 }
 
 void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, xm_catalog* cat) {
@@ -229,11 +225,8 @@ void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, 
         }
         radix_sort_pairs(ctx, &keys, &keys_alt, &vals, &vals_alt, n, SORT_LO_BIT, hi_bit, skip);
         dfree(ctx, keys_alt);
-        cat->id = static_cast<int64_t*>(dalloc(ctx, n * sizeof(int64_t)));
-        cat->x = static_cast<double*>(dalloc(ctx, n * sizeof(double)));
-        cat->y = static_cast<double*>(dalloc(ctx, n * sizeof(double)));
-        cat->z = static_cast<double*>(dalloc(ctx, n * sizeof(double)));
-        launch_gather(ctx, d_recs, vals, n, cat->id, cat->x, cat->y, cat->z);
+        cat->row = dalloc(ctx, n * sizeof(Row64));
+        launch_gather(ctx, d_recs, vals, n, static_cast<Row64*>(cat->row));
         dfree(ctx, vals);
         dfree(ctx, vals_alt);
         cat->key = keys;
@@ -461,21 +454,22 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
     return guarded(ctx, [&] {
         const uint64_t n = cat->n;
         if (n == 0) return;
-        auto d2h = [&](void* dst, const void* src, size_t bytes) {
-            if (dst) XM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        };
-        d2h(id, cat->id, n * 8);
-        d2h(x, cat->x, n * 8);
-        d2h(y, cat->y, n * 8);
-        d2h(z, cat->z, n * 8);
+        std::vector<Row64> rows(n);
+        XM_HIP(hipMemcpyAsync(rows.data(), cat->row, n * sizeof(Row64), hipMemcpyDeviceToHost,
+                              ctx->stream));
         std::vector<uint64_t> keys;
         if (zone) {
             keys.resize(n);
-            d2h(keys.data(), cat->key, n * 8);
+            XM_HIP(hipMemcpyAsync(keys.data(), cat->key, n * 8, hipMemcpyDeviceToHost, ctx->stream));
         }
         sync(ctx);
-        if (zone)
-            for (uint64_t i = 0; i < n; i++) zone[i] = key_zone(keys[i]);
+        for (uint64_t i = 0; i < n; i++) {
+            if (id) id[i] = rows[i].id;
+            if (x) x[i] = rows[i].x;
+            if (y) y[i] = rows[i].y;
+            if (z) z[i] = rows[i].z;
+            if (zone) zone[i] = key_zone(keys[i]);
+        }
     });
 }
 
@@ -489,6 +483,8 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     if (!res) return XM_ENOMEM;
     res->ctx = ctx;
     uint64_t* counter = nullptr;
+    Chunk* list = nullptr;
+    uint32_t ns = 0, nd = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     xm_match_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
@@ -504,6 +500,26 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         counter = static_cast<uint64_t*>(dalloc(ctx, sizeof(uint64_t)));
         XM_HIP(hipEventCreate(&ev0));
         XM_HIP(hipEventCreate(&ev1));
+        // split the chunks into sparse and dense lists for this radius
+        ns = 0;
+        nd = 0;
+        if (A->n_chunks) {
+            list = static_cast<Chunk*>(dalloc(ctx, A->n_chunks * sizeof(Chunk)));
+            uint32_t* cnts = static_cast<uint32_t*>(dalloc(ctx, 2 * sizeof(uint32_t)));
+            launch_fill_u32(ctx, cnts, 2, 0u);
+            XM_HIP(hipEventRecord(ev0, ctx->stream));
+            launch_classify(ctx, A, pp, list, cnts);
+            XM_HIP(hipEventRecord(ev1, ctx->stream));
+            uint32_t* pin = static_cast<uint32_t*>(ctx->pinned);
+            XM_HIP(hipMemcpyAsync(pin, cnts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
+            ns = pin[0];
+            nd = pin[1];
+            dfree(ctx, cnts);
+            float ms = 0.f;
+            XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            st_local.join_ms += ms;
+        }
         for (int attempt = 0;; attempt++) {
             res->cap = cap;
             res->id_a = static_cast<int64_t*>(dalloc(ctx, cap * sizeof(int64_t)));
@@ -511,7 +527,8 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             res->dist = static_cast<double*>(dalloc(ctx, cap * sizeof(double)));
             launch_fill_u64(ctx, counter, 1, 0);
             XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_pair_fp64(ctx, A, b, pp, counter, cap, res->id_a, res->id_b, res->dist);
+            launch_pair_fp64(ctx, A, b, pp, list, ns, nd, counter, cap, res->id_a, res->id_b,
+                             res->dist);
             XM_HIP(hipEventRecord(ev1, ctx->stream));
             uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
             XM_HIP(hipMemcpyAsync(pin, counter, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -522,6 +539,13 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             const uint64_t found = pin[0];
             if (found <= cap) {
                 res->n = found;
+                XM_HIP(hipEventRecord(ev0, ctx->stream));
+                launch_finalize_dist(ctx, res->dist, found);
+                XM_HIP(hipEventRecord(ev1, ctx->stream));
+                XM_HIP(hipEventSynchronize(ev1));
+                float ms2 = 0.f;
+                XM_HIP(hipEventElapsedTime(&ms2, ev0, ev1));
+                st_local.join_ms += ms2;
                 break;
             }
             // overflow: the counter holds the exact total; re-run into a buffer that fits
@@ -533,9 +557,11 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         }
         st_local.n_matches = res->n;
         st_local.n_chunks = A->n_chunks;
+        st_local.n_dense_chunks = nd;
     });
     (void)guarded(ctx, [&] {
         if (counter) dfree(ctx, counter);
+        if (list) dfree(ctx, list);
         if (ev0) XM_HIP(hipEventDestroy(ev0));
         if (ev1) XM_HIP(hipEventDestroy(ev1));
     });

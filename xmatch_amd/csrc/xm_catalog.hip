@@ -4,9 +4,10 @@
 // Replaces Sorter::Sort (BoostXmatch/BoostXmatch/Sorter.cu:141-185):
 //   GetZoneId + less_zonera (Sorter.cu:20-43)   -> k_make_keys (packed sort keys)
 //   lower_bound/upper_bound (Sorter.cu:161-164) -> k_zone_offsets
-//   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (ids + FP64 unit vectors,
-//                                                  radec2xyz of Worker.cu:31-40 done once
-//                                                  per source instead of per pair)
+//   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (one 32-byte row per source:
+//                                                  FP64 unit vector, radec2xyz of
+//                                                  Worker.cu:31-40 done once per source
+//                                                  instead of per pair, and the id)
 #include <algorithm>
 
 #include "xm_internal.h"
@@ -80,20 +81,17 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
 
 __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
                                                 const uint32_t* __restrict__ perm, uint32_t n,
-                                                int64_t* __restrict__ id, double* __restrict__ x,
-                                                double* __restrict__ y, double* __restrict__ z) {
+                                                Row64* __restrict__ rows) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = perm[i];
     const int64_t oid = recs[s].id;
     const double ra = recs[s].ra_deg;
     const double dec = recs[s].dec_deg;
-    double ux, uy, uz;
-    unit_vector(ra, dec, &ux, &uy, &uz);
-    id[i] = oid;
-    x[i] = ux;
-    y[i] = uy;
-    z[i] = uz;
+    Row64 r;
+    unit_vector(ra, dec, &r.x, &r.y, &r.z);
+    r.id = oid;
+    rows[i] = r;
 }
 
 // zoff[z - zmin] = first sorted row whose zone is >= z, for z in [zmin, zmax+1].
@@ -130,7 +128,7 @@ __global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__
         ch.start = r;
         ch.count = (r1 - r) < uint32_t(CHUNK_ROWS) ? (r1 - r) : uint32_t(CHUNK_ROWS);
         ch.zone = zmin + k;
-        ch.pad = 0;
+        ch.aq = 0;
         chunks[c] = ch;
     }
 }
@@ -155,12 +153,11 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
     });
 }
 
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, int64_t* id,
-                   double* x, double* y, double* z) {
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, Row64* rows) {
     if (n == 0) return;
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, perm,
-                           uint32_t(n), id, x, y, z);
+                           uint32_t(n), rows);
     });
 }
 

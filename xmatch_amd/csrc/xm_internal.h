@@ -62,10 +62,7 @@ struct xm_catalog {
     int32_t n_zones = 0;
     int32_t zmin = 0, zmax = -1;
     uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
-    int64_t* id = nullptr;
-    double* x = nullptr;         // FP64 unit vectors, SoA, sorted order
-    double* y = nullptr;
-    double* z = nullptr;
+    void* row = nullptr;         // xm::Row64[n]: FP64 unit vector + id, sorted order
     uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
     // A-side row chunks (built on first use as the A catalog of a join)
     void* chunks = nullptr;
@@ -86,11 +83,20 @@ namespace xm {
 constexpr int WAVE = 64;
 constexpr int CHUNK_ROWS = 64;   // A rows per wavefront in the pair kernel
 
+// One catalog row on the device: FP64 unit vector and id in one 32-byte line, so
+// a candidate B row is a single aligned fetch.
+struct Row64 {
+    double x, y, z;
+    int64_t id;
+};
+
+// A run of at most CHUNK_ROWS consecutive A rows of one zone.  aq is the ra
+// half-width of the zone's window in quanta (alpha_quanta), filled per join.
 struct Chunk {
     uint32_t start;
     uint32_t count;
     int32_t zone;
-    uint32_t pad;
+    int32_t aq;
 };
 
 // ---- memory (stream-ordered pool on the context stream)
@@ -123,8 +129,7 @@ void radix_sort_pairs(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint32_
 // ---- catalog kernels (xm_catalog.hip)
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad);
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n,
-                   int64_t* id, double* x, double* y, double* z);
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, Row64* rows);
 void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin,
                          int32_t zmax, uint32_t* zoff);
 void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts);
@@ -144,8 +149,15 @@ struct PairParams {
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
 };
+// Splits A's chunks into sparse ones (list[0, ns)) and dense ones (list[n-nd, n)),
+// filling each entry's aq; counts[0] = ns, counts[1] = nd (device, pre-zeroed).
+// dist[i] holds chord^2 on entry, the FP64 distance in arcsec on exit.
+void launch_finalize_dist(xm_ctx* ctx, double* dist, uint64_t n);
+void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
+                     uint32_t* counts);
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
-                      const PairParams& pp, uint64_t* counter, uint64_t cap, int64_t* out_a,
-                      int64_t* out_b, double* out_d);
+                      const PairParams& pp, const Chunk* list, uint32_t ns, uint32_t nd,
+                      uint64_t* counter, uint64_t cap, int64_t* out_a, int64_t* out_b,
+                      double* out_d);
 
 }  // namespace xm

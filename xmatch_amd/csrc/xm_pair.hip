@@ -1,21 +1,21 @@
-// xm_pair.hip -- the zone-join pair kernel (FP64 plain form).
+// xm_pair.hip -- the zone-join pair kernels (FP64 plain form).
 //
-// Replaces the per-zone-pair launch loop of Worker::Match and xmatch_kernel
-// (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  One launch covers the
-// whole join.  Each wavefront owns a chunk of up to 64 consecutive A rows of one
-// zone.  For each B zone z-nz..z+nz it finds, on the sorted packed keys, the rows
-// whose ra lies in the chunk's ra span widened by alpha(zone) (Common.cu:16-23),
-// the 0/360 seam split in two intervals, and tests them in one of two modes:
+// Replace the per-zone-pair launch loop of Worker::Match and xmatch_kernel
+// (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
+// into chunks of up to 64 consecutive rows of one zone; each wavefront owns one
+// chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
+// alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  A
+// classify pass splits the chunks of a join into two lists:
 //
-//  dense   one A row per lane (unit vector in registers); the window's B rows are
-//          walked in wave-uniform order, their unit vectors arriving as wide scalar
-//          (SGPR) loads, every lane testing its A row against each of them.
 //  sparse  the chunk's A unit vectors and ra quanta are staged in LDS; each lane
-//          takes one B row of the window, binary-searches the chunk for the A rows
-//          within +-alpha of its ra, and tests only those.  At zone height = radius
-//          a zone holds a few hundred sources over 360 deg, so a 64-row chunk spans
-//          tens of degrees while alpha is arcseconds: almost every dense test would
-//          be wasted.
+//          takes one B row, checks it against the window, binary-searches the
+//          chunk for the A rows within +-alpha of its ra and tests only those.  At
+//          zone height = radius a zone holds a few hundred sources over 360 deg, so
+//          a chunk spans tens of degrees while alpha is arcseconds: a dense test of
+//          every (A, B) pair of the window would almost always be wasted.
+//  dense   (pole rule, windows covering the circle, clustered data) one A row per
+//          lane in registers; the window's B rows are walked in wave-uniform order
+//          as wide scalar (SGPR) loads, every lane testing its A row against each.
 //
 // Predicate (Worker.cu:88-97 with the seam bug D2 corrected), on FP64 unit vectors:
 //   d2 = fma(dz, dz, fma(dy, dy, dx*dx))  <  (2 sin(r/2))^2
@@ -34,8 +34,9 @@ namespace {
 
 constexpr int PK_WAVES = 4;
 constexpr int PK_THREADS = PK_WAVES * WAVE;
-constexpr int STAGE = 256;
+constexpr int STAGE = 128;
 constexpr int UNROLL = 4;
+constexpr uint32_t ONE_PASS_ROWS = 4 * WAVE;  // B zones up to this size are read in one pass
 
 __device__ inline uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -96,8 +97,8 @@ struct Stage {
 };
 
 struct Out {
-    const int64_t* __restrict__ aid;
-    const int64_t* __restrict__ bid;
+    const Row64* __restrict__ arow;
+    const Row64* __restrict__ brow;
     unsigned long long* __restrict__ counter;
     uint64_t cap;
     int64_t* __restrict__ id_a;
@@ -114,9 +115,9 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
     for (uint32_t e = lane; e < st.n; e += WAVE) {
         const uint64_t slot = uint64_t(base) + e;
         if (slot < o.cap) {
-            o.id_a[slot] = o.aid[st.ra[e]];
-            o.id_b[slot] = o.bid[st.rb[e]];
-            o.dist[slot] = dist_arcsec(st.d2[e]);
+            o.id_a[slot] = o.arow[st.ra[e]].id;
+            o.id_b[slot] = o.brow[st.rb[e]].id;
+            o.dist[slot] = st.d2[e];  // chord^2; k_finalize_dist turns it into arcsec
         }
     }
     wave_lds_sync();
@@ -144,91 +145,106 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// Dense mode: B rows [j0, j1) against the lane's A row (registers).
-__device__ inline void dense_range(uint32_t j0, uint32_t j1, double ax, double ay, double az,
-                                   uint32_t row, double thr, const double* __restrict__ bx_,
-                                   const double* __restrict__ by_, const double* __restrict__ bz_,
-                                   uint32_t lane, Stage& st, const Out& o) {
-    uint32_t j = j0;
-    // UNROLL B rows per step: each coordinate arrives as one wide scalar load from a
-    // uniform base, and one branch covers the UNROLL hit masks
-    for (; j + UNROLL <= j1; j += UNROLL) {
-        const double* __restrict__ px = bx_ + size_t(j);
-        const double* __restrict__ py = by_ + size_t(j);
-        const double* __restrict__ pz = bz_ + size_t(j);
-        double d2[UNROLL];
-        uint64_t m[UNROLL];
-        uint64_t any = 0;
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            d2[u] = chord2(ax, ay, az, px[u], py[u], pz[u]);
-            m[u] = __ballot(d2[u] < thr);
-            any |= m[u];
-        }
-        if (any) {
-#pragma unroll
-            for (int u = 0; u < UNROLL; u++)
-                if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, st, o);
-        }
+// d2 -> 2 asin(sqrt(d2)/2) in arcsec, over the n result slots (FP64).
+__global__ __launch_bounds__(256) void k_finalize_dist(double* __restrict__ dist, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) dist[i] = dist_arcsec(dist[i]);
+}
+
+// ------------------------------------------------------------- classify ----
+__global__ __launch_bounds__(256) void k_classify(const Chunk* __restrict__ chunks, uint32_t n,
+                                                  const uint64_t* __restrict__ akey, PairParams pp,
+                                                  Chunk* __restrict__ list,
+                                                  uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const bool valid = i < n;
+    Chunk ch{0, 0, 0, 0};
+    bool dense = false;
+    if (valid) {
+        ch = chunks[i];
+        const uint64_t q_lo = akey[ch.start] & RA_MASK;
+        const uint64_t q_hi = akey[ch.start + ch.count - 1] & RA_MASK;
+        const int64_t aq = alpha_quanta(pp.theta_deg, pp.h_deg, ch.zone, pp.atan_sin_theta);
+        dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
+        ch.aq = int32_t(aq);  // WHOLE_ZONE (-1) only ever reaches the dense list
     }
-    for (; j < j1; j++) {
-        const double d2 = chord2(ax, ay, az, bx_[j], by_[j], bz_[j]);
-        const uint64_t m = __ballot(d2 < thr);
-        if (m) stage_hits(m, d2, row, j, lane, st, o);
+    const uint64_t md = __ballot(valid && dense);
+    const uint64_t ms = __ballot(valid && !dense);
+    uint32_t bd = 0, bs = 0;
+    if (lane == 0) {
+        if (md) bd = atomicAdd(&counts[1], uint32_t(__popcll(md)));
+        if (ms) bs = atomicAdd(&counts[0], uint32_t(__popcll(ms)));
+    }
+    bd = __shfl(bd, 0, WAVE);
+    bs = __shfl(bs, 0, WAVE);
+    if (valid) {
+        if (dense) list[n - 1 - (bd + mbcnt64(md))] = ch;
+        else list[bs + mbcnt64(ms)] = ch;
     }
 }
 
-// Sparse mode: one B row of [j0, j1) per lane, tested against the chunk rows whose
-// ra quantum lies within +-aq of its own (shifted into the chunk's frame).
-// s_aq holds the chunk's ra quanta ascending, padded with INT_MAX to 64 entries.
-__device__ inline void sparse_range(uint32_t j0, uint32_t j1, int32_t shift, int32_t aq,
-                                    uint32_t start, uint32_t cnt, const int32_t* s_aq,
-                                    const double* s_ax, const double* s_ay, const double* s_az,
-                                    double thr, const uint64_t* __restrict__ bkey,
-                                    const double* __restrict__ bx_, const double* __restrict__ by_,
-                                    const double* __restrict__ bz_, uint32_t lane, Stage& st,
-                                    const Out& o) {
-    for (uint32_t jb = j0; jb < j1; jb += WAVE) {
-        const uint32_t j = jb + lane;
-        const bool valid = j < j1;
-        const int32_t qb = valid ? int32_t(bkey[j] & RA_MASK) + shift : 0;
-        const int32_t lo_q = qb - aq, hi_q = qb + aq;
-        // c = number of chunk rows with quantum < lo_q (branchless search of 64)
-        uint32_t c = 0;
-#pragma unroll
-        for (uint32_t step = WAVE / 2; step >= 1; step >>= 1)
-            if (s_aq[c + step - 1] < lo_q) c += step;
-        if (s_aq[c] < lo_q) c++;
-        bool have_b = false;
-        double bx = 0.0, by = 0.0, bz = 0.0;
-        for (uint32_t k = 0;; k++) {
-            const uint32_t cc = c + k;
-            const bool act = valid && cc < cnt && s_aq[cc < uint32_t(WAVE) ? cc : 0] <= hi_q;
-            if (!__ballot(act)) break;
-            double d2 = 0.0;
-            bool hit = false;
-            if (act) {
-                if (!have_b) {
-                    bx = bx_[j];
-                    by = by_[j];
-                    bz = bz_[j];
-                    have_b = true;
-                }
-                d2 = chord2(s_ax[cc], s_ay[cc], s_az[cc], bx, by, bz);
-                hit = d2 < thr;
+// --------------------------------------------------------------- sparse ----
+struct SparseCtx {
+    uint32_t start, cnt;
+    int32_t aq;
+    const int32_t* s_aq;
+    const double *s_ax, *s_ay, *s_az;
+    double thr;
+};
+
+// One B row per lane (row j, key); tested against the chunk rows within +-aq of
+// its ra if it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).
+__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key, int ni,
+                                  const int32_t* lq, const int32_t* hq, const int32_t* shift,
+                                  const Row64* __restrict__ brow, uint32_t lane, Stage& st,
+                                  const Out& o) {
+    const int32_t q = int32_t(key & RA_MASK);
+    bool in = false;
+    int32_t qb = 0;
+    if (valid) {
+        for (int t = 0; t < ni; t++)
+            if (q >= lq[t] && q <= hq[t]) {
+                in = true;
+                qb = q + shift[t];
             }
-            const uint64_t m = __ballot(hit);
-            if (m) stage_hits(m, d2, start + cc, j, lane, st, o);
+    }
+    if (!__ballot(in)) return;
+    const int32_t lo_q = qb - x.aq, hi_q = qb + x.aq;
+    // c = number of chunk rows with quantum < lo_q (branchless search of 64)
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t step = WAVE / 2; step >= 1; step >>= 1)
+        if (x.s_aq[c + step - 1] < lo_q) c += step;
+    if (x.s_aq[c] < lo_q) c++;
+    bool have_b = false;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    for (uint32_t k = 0;; k++) {
+        const uint32_t cc = c + k;
+        const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(WAVE) ? cc : 0] <= hi_q;
+        if (!__ballot(act)) break;
+        double d2 = 0.0;
+        bool hit = false;
+        if (act) {
+            if (!have_b) {
+                const Row64 b = brow[j];
+                bx = b.x;
+                by = b.y;
+                bz = b.z;
+                have_b = true;
+            }
+            d2 = chord2(x.s_ax[cc], x.s_ay[cc], x.s_az[cc], bx, by, bz);
+            hit = d2 < x.thr;
         }
+        const uint64_t m = __ballot(hit);
+        if (m) stage_hits(m, d2, x.start + cc, j, lane, st, o);
     }
 }
 
-__global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
-    const Chunk* __restrict__ chunks, uint32_t n_chunks, const uint64_t* __restrict__ akey,
-    const double* __restrict__ ax_, const double* __restrict__ ay_, const double* __restrict__ az_,
-    const int64_t* __restrict__ aid, const uint64_t* __restrict__ bkey,
-    const double* __restrict__ bx_, const double* __restrict__ by_, const double* __restrict__ bz_,
-    const int64_t* __restrict__ bid, const uint32_t* __restrict__ bzoff, int32_t bzmin,
+__global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
+    const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
+    const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
+    const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
     int32_t bzmax, PairParams pp, unsigned long long* __restrict__ counter, uint64_t cap,
     int64_t* __restrict__ out_a, int64_t* __restrict__ out_b, double* __restrict__ out_d) {
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
@@ -242,35 +258,112 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
     const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
-    if (c >= n_chunks) return;  // whole wavefront; the kernel has no block barrier
+    if (c >= n_list) return;  // whole wavefront; the kernel has no block barrier
 
-    const uint32_t start = uni(chunks[c].start);
-    const uint32_t cnt = uni(chunks[c].count);
-    const int32_t za = int32_t(uni(uint32_t(chunks[c].zone)));
+    const uint32_t start = uni(list[c].start);
+    const uint32_t cnt = uni(list[c].count);
+    const int32_t za = int32_t(uni(uint32_t(list[c].zone)));
+    const int32_t aq = int32_t(uni(uint32_t(list[c].aq)));
 
-    const bool act = lane < cnt;
-    const uint32_t row = start + lane;
-    double ax = 10.0, ay = 10.0, az = 10.0;  // inactive lanes sit far off the sphere
     int32_t qa = INT_MAX;
-    if (act) {
-        ax = ax_[row];
-        ay = ay_[row];
-        az = az_[row];
-        qa = int32_t(akey[row] & RA_MASK);
+    double ax = 10.0, ay = 10.0, az = 10.0;
+    if (lane < cnt) {
+        const Row64 a = arow[start + lane];
+        ax = a.x;
+        ay = a.y;
+        az = a.z;
+        qa = int32_t(akey[start + lane] & RA_MASK);
     }
     s_aq[w][lane] = qa;
     s_ax[w][lane] = ax;
     s_ay[w][lane] = ay;
     s_az[w][lane] = az;
     wave_lds_sync();
+    const uint64_t q_lo = uint32_t(__builtin_amdgcn_readlane(qa, 0));
+    const uint64_t q_hi = uint32_t(__builtin_amdgcn_readlane(qa, int(cnt - 1)));
 
+    const SparseCtx x{start, cnt, aq, s_aq[w], s_ax[w], s_ay[w], s_az[w], pp.chord2};
+    Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
+    const Out o{arow, brow, counter, cap, out_a, out_b, out_d};
+
+    for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
+        const int32_t zb = za + dz;
+        if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
+        const uint32_t b0 = uni(bzoff[zb - bzmin]);
+        const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
+        if (b0 >= b1) continue;
+        uint64_t lo[2], hi[2];
+        int32_t shift[2], lq[2], hq[2];
+        const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
+        for (int t = 0; t < ni; t++) {
+            lq[t] = int32_t(lo[t] & RA_MASK);
+            hq[t] = int32_t(hi[t] & RA_MASK);
+        }
+        if (b1 - b0 <= ONE_PASS_ROWS) {
+            // the whole zone in one round trip; rows outside the window drop out
+            uint64_t kk[ONE_PASS_ROWS / WAVE];
+#pragma unroll
+            for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
+                const uint32_t j = b0 + k * WAVE + lane;
+                kk[k] = j < b1 ? bkey[j] : 0;
+            }
+#pragma unroll 1
+            for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
+                if (b0 + k * WAVE >= b1) break;
+                const uint32_t j = b0 + k * WAVE + lane;
+                sparse_row(x, j, j < b1, kk[k], ni, lq, hq, shift, brow, lane, st, o);
+            }
+        } else {
+            for (int t = 0; t < ni; t++) {
+                const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
+                const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
+                for (uint32_t jb = j0; jb < j1; jb += WAVE) {
+                    const uint32_t j = jb + lane;
+                    const bool valid = j < j1;
+                    const uint64_t key = valid ? bkey[j] : 0;
+                    sparse_row(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], brow, lane, st, o);
+                }
+            }
+        }
+    }
+    if (st.n) flush_stage(st, lane, o);
+}
+
+// ---------------------------------------------------------------- dense ----
+__global__ __launch_bounds__(PK_THREADS) void k_pair_dense(
+    const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
+    const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
+    const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
+    int32_t bzmax, PairParams pp, unsigned long long* __restrict__ counter, uint64_t cap,
+    int64_t* __restrict__ out_a, int64_t* __restrict__ out_b, double* __restrict__ out_d) {
+    __shared__ uint32_t s_ra[PK_WAVES][STAGE];
+    __shared__ uint32_t s_rb[PK_WAVES][STAGE];
+    __shared__ double s_d2[PK_WAVES][STAGE];
+
+    const uint32_t lane = lane_id();
+    const uint32_t w = uni(threadIdx.x / WAVE);
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
+    if (c >= n_list) return;  // whole wavefront; the kernel has no block barrier
+
+    const uint32_t start = uni(list[c].start);
+    const uint32_t cnt = uni(list[c].count);
+    const int32_t za = int32_t(uni(uint32_t(list[c].zone)));
+    const int64_t aq = int64_t(int32_t(uni(uint32_t(list[c].aq))));
+
+    const uint32_t row = start + lane;
+    double ax = 10.0, ay = 10.0, az = 10.0;  // inactive lanes sit far off the sphere
+    if (lane < cnt) {
+        const Row64 a = arow[row];
+        ax = a.x;
+        ay = a.y;
+        az = a.z;
+    }
     const uint64_t q_lo = akey[start] & RA_MASK;
     const uint64_t q_hi = akey[start + cnt - 1] & RA_MASK;
-    const int64_t aq = alpha_quanta(pp.theta_deg, pp.h_deg, za, pp.atan_sin_theta);
-    const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
+    const double thr = pp.chord2;
 
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o{aid, bid, counter, cap, out_a, out_b, out_d};
+    const Out o{arow, brow, counter, cap, out_a, out_b, out_d};
 
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
@@ -284,11 +377,31 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
         for (int t = 0; t < ni; t++) {
             const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
             const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
-            if (dense)
-                dense_range(j0, j1, ax, ay, az, row, pp.chord2, bx_, by_, bz_, lane, st, o);
-            else
-                sparse_range(j0, j1, shift[t], int32_t(aq), start, cnt, s_aq[w], s_ax[w], s_ay[w],
-                             s_az[w], pp.chord2, bkey, bx_, by_, bz_, lane, st, o);
+            uint32_t j = j0;
+            // UNROLL B rows per step: the rows arrive as wide scalar loads from a
+            // uniform base, and one branch covers the UNROLL hit masks
+            for (; j + UNROLL <= j1; j += UNROLL) {
+                const Row64* __restrict__ pb = brow + size_t(j);
+                double d2[UNROLL];
+                uint64_t m[UNROLL];
+                uint64_t any = 0;
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    d2[u] = chord2(ax, ay, az, pb[u].x, pb[u].y, pb[u].z);
+                    m[u] = __ballot(d2[u] < thr);
+                    any |= m[u];
+                }
+                if (any) {
+#pragma unroll
+                    for (int u = 0; u < UNROLL; u++)
+                        if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, st, o);
+                }
+            }
+            for (; j < j1; j++) {
+                const double d2 = chord2(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
+                const uint64_t m = __ballot(d2 < thr);
+                if (m) stage_hits(m, d2, row, j, lane, st, o);
+            }
         }
     }
     if (st.n) flush_stage(st, lane, o);
@@ -296,17 +409,47 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
 
 }  // namespace
 
-void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                      uint64_t* counter, uint64_t cap, int64_t* out_a, int64_t* out_b,
-                      double* out_d) {
-    if (A->n_chunks == 0 || B->n == 0) return;
-    const uint64_t nblk = (A->n_chunks + PK_WAVES - 1) / PK_WAVES;
-    launch_timed(ctx, "pair_fp64", [&] {
-        hipLaunchKernelGGL(k_pair_fp64, dim3(uint32_t(nblk)), dim3(PK_THREADS), 0, ctx->stream,
-                           (const Chunk*)A->chunks, uint32_t(A->n_chunks), A->key, A->x, A->y, A->z,
-                           A->id, B->key, B->x, B->y, B->z, B->id, B->zoff, B->zmin, B->zmax, pp,
-                           (unsigned long long*)counter, cap, out_a, out_b, out_d);
+void launch_finalize_dist(xm_ctx* ctx, double* dist, uint64_t n) {
+    if (n == 0) return;
+    launch_timed(ctx, "finalize_dist", [&] {
+        hipLaunchKernelGGL(k_finalize_dist, dim3(uint32_t((n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dist, n);
     });
+}
+
+void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
+                     uint32_t* counts) {
+    if (A->n_chunks == 0) return;
+    const uint32_t nblk = uint32_t((A->n_chunks + 255) / 256);
+    launch_timed(ctx, "classify", [&] {
+        hipLaunchKernelGGL(k_classify, dim3(nblk), dim3(256), 0, ctx->stream,
+                           (const Chunk*)A->chunks, uint32_t(A->n_chunks), A->key, pp, list, counts);
+    });
+}
+
+void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
+                      const Chunk* list, uint32_t ns, uint32_t nd, uint64_t* counter, uint64_t cap,
+                      int64_t* out_a, int64_t* out_b, double* out_d) {
+    if (B->n == 0) return;
+    const Row64* arow = static_cast<const Row64*>(A->row);
+    const Row64* brow = static_cast<const Row64*>(B->row);
+    if (ns) {
+        const uint32_t nblk = (ns + PK_WAVES - 1) / PK_WAVES;
+        launch_timed(ctx, "pair_sparse", [&] {
+            hipLaunchKernelGGL(k_pair_sparse, dim3(nblk), dim3(PK_THREADS), 0, ctx->stream, list,
+                               ns, A->key, arow, B->key, brow, B->zoff, B->zmin, B->zmax, pp,
+                               (unsigned long long*)counter, cap, out_a, out_b, out_d);
+        });
+    }
+    if (nd) {
+        const uint32_t nblk = (nd + PK_WAVES - 1) / PK_WAVES;
+        launch_timed(ctx, "pair_dense", [&] {
+            hipLaunchKernelGGL(k_pair_dense, dim3(nblk), dim3(PK_THREADS), 0, ctx->stream,
+                               list + (A->n_chunks - nd), nd, A->key, arow, B->key, brow, B->zoff,
+                               B->zmin, B->zmax, pp, (unsigned long long*)counter, cap, out_a,
+                               out_b, out_d);
+        });
+    }
 }
 
 }  // namespace xm
