@@ -20,6 +20,8 @@ using namespace xm;
 namespace {
 
 thread_local std::string g_tls_error = "no error";
+// pinned host scratch: region counters and their prefix sums
+constexpr size_t PINNED_BYTES = size_t(MAX_REGIONS) * (REGION_STRIDE + 1) * sizeof(uint64_t);
 
 template <typename F>
 xm_status guarded(xm_ctx* ctx, F&& f) {
@@ -260,7 +262,7 @@ xm_status xm_ctx_create(int device, xm_ctx** out) {
         XM_HIP(hipDeviceGetDefaultMemPool(&pool, device));
         uint64_t thr = UINT64_MAX;
         XM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
-        XM_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        XM_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
     });
     if (st != XM_OK) {
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -482,7 +484,7 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     xm_result* res = new (std::nothrow) xm_result();
     if (!res) return XM_ENOMEM;
     res->ctx = ctx;
-    uint64_t* counter = nullptr;
+    uint64_t* counters = nullptr;
     Chunk* list = nullptr;
     uint32_t ns = 0, nd = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -497,7 +499,6 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         const PairParams pp = make_params(ctx, A, radius_deg);
         uint64_t cap = capacity_hint ? capacity_hint : 2 * std::max(a->n, b->n);
         cap = std::max<uint64_t>(cap, 1024);
-        counter = static_cast<uint64_t*>(dalloc(ctx, sizeof(uint64_t)));
         XM_HIP(hipEventCreate(&ev0));
         XM_HIP(hipEventCreate(&ev1));
         // split the chunks into sparse and dense lists for this radius
@@ -520,39 +521,64 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
             st_local.join_ms += ms;
         }
+        // hits go to n_regions append regions (one per group of blocks), then are
+        // packed into the contiguous result
+        const uint32_t nblk = std::max(pair_blocks(ns), pair_blocks(nd));
+        RegionOut ro;
+        ro.n_regions = std::max<uint32_t>(1, std::min<uint32_t>(MAX_REGIONS, nblk));
+        ro.region_cap = std::max<uint64_t>(1, (cap + ro.n_regions - 1) / ro.n_regions);
+        ro.counters = static_cast<uint64_t*>(
+            dalloc(ctx, size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t)));
+        counters = ro.counters;
+        uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
         for (int attempt = 0;; attempt++) {
-            res->cap = cap;
-            res->id_a = static_cast<int64_t*>(dalloc(ctx, cap * sizeof(int64_t)));
-            res->id_b = static_cast<int64_t*>(dalloc(ctx, cap * sizeof(int64_t)));
-            res->dist = static_cast<double*>(dalloc(ctx, cap * sizeof(double)));
-            launch_fill_u64(ctx, counter, 1, 0);
+            ro.id_a = static_cast<int64_t*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
+            ro.id_b = static_cast<int64_t*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
+            ro.d2 = static_cast<double*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
+            launch_fill_u64(ctx, ro.counters, size_t(ro.n_regions) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_pair_fp64(ctx, A, b, pp, list, ns, nd, counter, cap, res->id_a, res->id_b,
-                             res->dist);
+            launch_pair_fp64(ctx, A, b, pp, list, ns, nd, ro);
             XM_HIP(hipEventRecord(ev1, ctx->stream));
-            uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
-            XM_HIP(hipMemcpyAsync(pin, counter, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+            XM_HIP(hipMemcpyAsync(pin, ro.counters,
+                                  size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
             sync(ctx);
             float ms = 0.f;
             XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
             st_local.join_ms += ms;
-            const uint64_t found = pin[0];
-            if (found <= cap) {
-                res->n = found;
+            uint64_t total = 0, most = 0;
+            uint64_t* prefix_h = pin + size_t(ro.n_regions) * REGION_STRIDE;
+            for (uint32_t r = 0; r < ro.n_regions; r++) {
+                const uint64_t c = pin[size_t(r) * REGION_STRIDE];
+                prefix_h[r] = total;
+                total += c;
+                most = std::max(most, c);
+            }
+            if (most <= ro.region_cap) {
+                res->n = total;
+                res->cap = total;
+                res->id_a = static_cast<int64_t*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
+                res->id_b = static_cast<int64_t*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
+                res->dist = static_cast<double*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
+                uint64_t* prefix_d = static_cast<uint64_t*>(dalloc(ctx, ro.n_regions * 8));
+                XM_HIP(hipMemcpyAsync(prefix_d, prefix_h, ro.n_regions * 8, hipMemcpyHostToDevice,
+                                      ctx->stream));
                 XM_HIP(hipEventRecord(ev0, ctx->stream));
-                launch_finalize_dist(ctx, res->dist, found);
+                launch_compact(ctx, ro, prefix_d, res->id_a, res->id_b, res->dist);
                 XM_HIP(hipEventRecord(ev1, ctx->stream));
+                dfree(ctx, prefix_d);
+                dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
                 XM_HIP(hipEventSynchronize(ev1));
                 float ms2 = 0.f;
                 XM_HIP(hipEventElapsedTime(&ms2, ev0, ev1));
                 st_local.join_ms += ms2;
                 break;
             }
-            // overflow: the counter holds the exact total; re-run into a buffer that fits
+            // overflow: every counter holds its region's exact total; re-run with
+            // regions that fit the fullest one
             st_local.n_retries++;
-            dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist);
-            res->id_a = nullptr; res->id_b = nullptr; res->dist = nullptr;
-            cap = found;
+            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
+            ro.region_cap = most;
             if (attempt > 2) throw std::runtime_error("result size did not converge");
         }
         st_local.n_matches = res->n;
@@ -560,7 +586,7 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         st_local.n_dense_chunks = nd;
     });
     (void)guarded(ctx, [&] {
-        if (counter) dfree(ctx, counter);
+        if (counters) dfree(ctx, counters);
         if (list) dfree(ctx, list);
         if (ev0) XM_HIP(hipEventDestroy(ev0));
         if (ev1) XM_HIP(hipEventDestroy(ev1));

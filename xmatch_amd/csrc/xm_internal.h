@@ -151,13 +151,29 @@ struct PairParams {
 };
 // Splits A's chunks into sparse ones (list[0, ns)) and dense ones (list[n-nd, n)),
 // filling each entry's aq; counts[0] = ns, counts[1] = nd (device, pre-zeroed).
-// dist[i] holds chord^2 on entry, the FP64 distance in arcsec on exit.
-void launch_finalize_dist(xm_ctx* ctx, double* dist, uint64_t n);
 void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
                      uint32_t* counts);
+// Hits are appended to n_regions regions of region_cap slots; block b appends to
+// region b % n_regions through counters[region * REGION_STRIDE] (one 128-byte line
+// per counter, so the waves of a join never queue on one address).  Counters count
+// every hit; slots past region_cap are dropped and the caller re-runs.
+constexpr uint32_t MAX_REGIONS = 1024;
+constexpr uint32_t REGION_STRIDE = 16;
+struct RegionOut {
+    uint64_t* counters;
+    uint32_t n_regions;
+    uint64_t region_cap;
+    int64_t* id_a;   // [n_regions * region_cap]
+    int64_t* id_b;
+    double* d2;      // chord^2 of each hit
+};
+uint32_t pair_blocks(uint32_t n_chunks);
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
                       const PairParams& pp, const Chunk* list, uint32_t ns, uint32_t nd,
-                      uint64_t* counter, uint64_t cap, int64_t* out_a, int64_t* out_b,
-                      double* out_d);
+                      const RegionOut& ro);
+// Packs the regions into contiguous arrays (region r's hits at prefix[r]) and
+// converts chord^2 to the FP64 distance in arcsec.
+void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
+                    int64_t* id_b, double* dist);
 
 }  // namespace xm

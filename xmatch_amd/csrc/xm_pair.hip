@@ -20,10 +20,13 @@
 // Predicate (Worker.cu:88-97 with the seam bug D2 corrected), on FP64 unit vectors:
 //   d2 = fma(dz, dz, fma(dy, dy, dx*dx))  <  (2 sin(r/2))^2
 // Hits are compacted with ballot + mbcnt into a per-wave LDS stage; a full stage
-// (or the end of the chunk) costs one global atomic on the 64-bit append counter.
-// Slots beyond the buffer capacity are counted but not written; the host then
-// grows the buffer and re-runs, so nothing is ever truncated (Worker.cu:147-148,
-// 384-387 dropped them).
+// (or the end of the chunk) costs one global atomic on the 64-bit append counter
+// of the block's output region.  A join's ~10^6 wavefronts spread over up to 1024
+// region counters, each on its own 128-byte line: same-address device atomics
+// serialise at ~10 ns each, so one shared counter would cost milliseconds.  Slots
+// beyond a region's capacity are counted but not written; the host then grows the
+// regions and re-runs, so nothing is ever truncated (Worker.cu:147-148, 384-387
+// dropped them).  k_compact packs the regions into the contiguous result.
 #include <climits>
 
 #include "xm_internal.h"
@@ -99,14 +102,21 @@ struct Stage {
 struct Out {
     const Row64* __restrict__ arow;
     const Row64* __restrict__ brow;
-    unsigned long long* __restrict__ counter;
-    uint64_t cap;
-    int64_t* __restrict__ id_a;
+    unsigned long long* __restrict__ counter;  // this block's region counter
+    uint64_t region_cap;
+    int64_t* __restrict__ id_a;                // this block's region
     int64_t* __restrict__ id_b;
-    double* __restrict__ dist;
+    double* __restrict__ d2;
 };
 
-// Write the wave's staged hits to the result buffer: one atomic per call.
+__device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro) {
+    const uint32_t r = blockIdx.x % ro.n_regions;
+    const uint64_t base = uint64_t(r) * ro.region_cap;
+    return Out{arow, brow, (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE),
+               ro.region_cap, ro.id_a + base, ro.id_b + base, ro.d2 + base};
+}
+
+// Write the wave's staged hits to its region: one atomic per call.
 __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
     wave_lds_sync();
     unsigned long long base = 0;
@@ -114,10 +124,10 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
     base = __shfl(base, 0, WAVE);
     for (uint32_t e = lane; e < st.n; e += WAVE) {
         const uint64_t slot = uint64_t(base) + e;
-        if (slot < o.cap) {
+        if (slot < o.region_cap) {
             o.id_a[slot] = o.arow[st.ra[e]].id;
             o.id_b[slot] = o.brow[st.rb[e]].id;
-            o.dist[slot] = st.d2[e];  // chord^2; k_finalize_dist turns it into arcsec
+            o.d2[slot] = st.d2[e];
         }
     }
     wave_lds_sync();
@@ -145,10 +155,26 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// d2 -> 2 asin(sqrt(d2)/2) in arcsec, over the n result slots (FP64).
-__global__ __launch_bounds__(256) void k_finalize_dist(double* __restrict__ dist, uint64_t n) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) dist[i] = dist_arcsec(dist[i]);
+// Region r's hits [r*region_cap, +count_r) -> [prefix[r], +count_r), with chord^2
+// turned into 2 asin(sqrt(d2)/2) in arcsec (FP64).  One block per region.
+__global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ counters,
+                                                 uint64_t region_cap,
+                                                 const int64_t* __restrict__ ta,
+                                                 const int64_t* __restrict__ tb,
+                                                 const double* __restrict__ td,
+                                                 const uint64_t* __restrict__ prefix,
+                                                 int64_t* __restrict__ id_a,
+                                                 int64_t* __restrict__ id_b,
+                                                 double* __restrict__ dist) {
+    const uint32_t r = blockIdx.x;
+    uint64_t n = counters[size_t(r) * REGION_STRIDE];
+    n = n < region_cap ? n : region_cap;
+    const uint64_t src = uint64_t(r) * region_cap, dst = prefix[r];
+    for (uint64_t i = threadIdx.x; i < n; i += 256) {
+        id_a[dst + i] = ta[src + i];
+        id_b[dst + i] = tb[src + i];
+        dist[dst + i] = dist_arcsec(td[src + i]);
+    }
 }
 
 // ------------------------------------------------------------- classify ----
@@ -245,8 +271,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
     const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
     const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
     const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
-    int32_t bzmax, PairParams pp, unsigned long long* __restrict__ counter, uint64_t cap,
-    int64_t* __restrict__ out_a, int64_t* __restrict__ out_b, double* __restrict__ out_d) {
+    int32_t bzmax, PairParams pp, RegionOut ro) {
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
@@ -284,7 +309,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
 
     const SparseCtx x{start, cnt, aq, s_aq[w], s_ax[w], s_ay[w], s_az[w], pp.chord2};
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o{arow, brow, counter, cap, out_a, out_b, out_d};
+    const Out o = make_out(arow, brow, ro);
 
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
@@ -334,8 +359,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_dense(
     const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
     const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
     const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
-    int32_t bzmax, PairParams pp, unsigned long long* __restrict__ counter, uint64_t cap,
-    int64_t* __restrict__ out_a, int64_t* __restrict__ out_b, double* __restrict__ out_d) {
+    int32_t bzmax, PairParams pp, RegionOut ro) {
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
@@ -363,7 +387,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_dense(
     const double thr = pp.chord2;
 
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o{arow, brow, counter, cap, out_a, out_b, out_d};
+    const Out o = make_out(arow, brow, ro);
 
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
@@ -409,13 +433,17 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_dense(
 
 }  // namespace
 
-void launch_finalize_dist(xm_ctx* ctx, double* dist, uint64_t n) {
-    if (n == 0) return;
-    launch_timed(ctx, "finalize_dist", [&] {
-        hipLaunchKernelGGL(k_finalize_dist, dim3(uint32_t((n + 255) / 256)), dim3(256), 0,
-                           ctx->stream, dist, n);
+void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
+                    int64_t* id_b, double* dist) {
+    launch_timed(ctx, "compact", [&] {
+        hipLaunchKernelGGL(k_compact, dim3(ro.n_regions), dim3(256), 0, ctx->stream,
+                           (const uint64_t*)ro.counters, ro.region_cap, (const int64_t*)ro.id_a,
+                           (const int64_t*)ro.id_b, (const double*)ro.d2, prefix, id_a, id_b,
+                           dist);
     });
 }
+
+uint32_t pair_blocks(uint32_t n_chunks) { return (n_chunks + PK_WAVES - 1) / PK_WAVES; }
 
 void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
                      uint32_t* counts) {
@@ -428,26 +456,22 @@ void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chu
 }
 
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                      const Chunk* list, uint32_t ns, uint32_t nd, uint64_t* counter, uint64_t cap,
-                      int64_t* out_a, int64_t* out_b, double* out_d) {
+                      const Chunk* list, uint32_t ns, uint32_t nd, const RegionOut& ro) {
     if (B->n == 0) return;
     const Row64* arow = static_cast<const Row64*>(A->row);
     const Row64* brow = static_cast<const Row64*>(B->row);
     if (ns) {
-        const uint32_t nblk = (ns + PK_WAVES - 1) / PK_WAVES;
         launch_timed(ctx, "pair_sparse", [&] {
-            hipLaunchKernelGGL(k_pair_sparse, dim3(nblk), dim3(PK_THREADS), 0, ctx->stream, list,
-                               ns, A->key, arow, B->key, brow, B->zoff, B->zmin, B->zmax, pp,
-                               (unsigned long long*)counter, cap, out_a, out_b, out_d);
+            hipLaunchKernelGGL(k_pair_sparse, dim3(pair_blocks(ns)), dim3(PK_THREADS), 0,
+                               ctx->stream, list, ns, A->key, arow, B->key, brow, B->zoff, B->zmin,
+                               B->zmax, pp, ro);
         });
     }
     if (nd) {
-        const uint32_t nblk = (nd + PK_WAVES - 1) / PK_WAVES;
         launch_timed(ctx, "pair_dense", [&] {
-            hipLaunchKernelGGL(k_pair_dense, dim3(nblk), dim3(PK_THREADS), 0, ctx->stream,
-                               list + (A->n_chunks - nd), nd, A->key, arow, B->key, brow, B->zoff,
-                               B->zmin, B->zmax, pp, (unsigned long long*)counter, cap, out_a,
-                               out_b, out_d);
+            hipLaunchKernelGGL(k_pair_dense, dim3(pair_blocks(nd)), dim3(PK_THREADS), 0,
+                               ctx->stream, list + (A->n_chunks - nd), nd, A->key, arow, B->key,
+                               brow, B->zoff, B->zmin, B->zmax, pp, ro);
         });
     }
 }
