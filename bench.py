@@ -40,7 +40,7 @@ BYTES_PER_SOURCE = {
 }
 KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
            "scan_apply", "gather", "zone_offsets", "chunk_counts", "chunk_fill", "fill",
-           "classify", "pair_sparse", "pair_dense", "finalize_dist"]
+           "classify", "pair_sparse", "pair_dense", "compact"]
 
 
 def log(msg):
