@@ -260,3 +260,28 @@ def test_join_mode_does_not_change_the_set(gpu_ctx, dense_min):
     finally:
         gpu_ctx.set_param("dense_min", old)
     assert_same(got, ref)
+
+
+@pytest.mark.parametrize("ra_bits", [6, 12, 18])
+def test_coarse_ra_keys_do_not_change_the_set(gpu_ctx, ra_bits):
+    # keys that floor ra to 2^-ra_bits turn (as 10^8-row catalogs do) only widen
+    # the windows; A and B may even use different granularities
+    A, B, r = synth.config_small_patch(seed=4, n_a=60_000, n_b=60_000)
+    ref, _ = O.match(A, B, r)
+    old = gpu_ctx.get_param("key_ra_bits")
+    try:
+        gpu_ctx.set_param("key_ra_bits", ra_bits)
+        ca = gpu_ctx.catalog(A, r)
+        gpu_ctx.set_param("key_ra_bits", 24 if ra_bits != 24 else 20)
+        cb = gpu_ctx.catalog(B, r)
+    finally:
+        gpu_ctx.set_param("key_ra_bits", old)
+    res = gpu_ctx.match(ca, cb, r)
+    got = res.to_host()
+    res.close()
+    res = gpu_ctx.match(cb, ca, r)  # the other way round: A coarse on the B side
+    got2 = res.to_host()
+    res.close(); ca.close(); cb.close()
+    assert_same(got, ref)
+    o = np.lexsort((got2[0], got2[1]))
+    assert np.array_equal(got2[1][o], ref[0]) and np.array_equal(got2[0][o], ref[1])

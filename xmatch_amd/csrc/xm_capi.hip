@@ -81,12 +81,14 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     A->n_chunks = total;
 }
 
-PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, double radius_deg) {
+PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
+                       double radius_deg) {
     PairParams pp;
     pp.theta_deg = radius_deg;
     pp.h_deg = A->h_deg;
     pp.atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
     pp.chord2 = chord2_of_radius(radius_deg);
+    pp.gran = std::max(A->ra_gran, B->ra_gran);
     pp.nz = int32_t(std::ceil(radius_deg / A->h_deg));  // Worker.cu:313
     pp.n_zones = A->n_zones;
     pp.dense_min = ctx->dense_min;
@@ -192,15 +194,16 @@ void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, 
     cat->zmin = 0;
     cat->zmax = -1;
     if (n > 0) {
-        uint64_t* keys = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-        uint64_t* keys_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-        uint32_t* vals = static_cast<uint32_t*>(dalloc(ctx, n * sizeof(uint32_t)));
-        uint32_t* vals_alt = static_cast<uint32_t*>(dalloc(ctx, n * sizeof(uint32_t)));
+        PackLayout pl = pack_layout(n, cat->n_zones);
+        if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
+        cat->ra_gran = int64_t(1) << (RA_BITS - pl.ra_bits);
+        uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+        uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
         int32_t* small = static_cast<int32_t*>(dalloc(ctx, 4 * sizeof(int32_t)));
         launch_fill_i32(ctx, small, 1, INT_MAX);
         launch_fill_i32(ctx, small + 1, 1, -1);
         launch_fill_i32(ctx, small + 2, 1, 0);
-        launch_make_keys(ctx, d_recs, n, h_deg, cat->n_zones, keys, vals, small,
+        launch_make_keys(ctx, d_recs, n, h_deg, cat->n_zones, pl, words, small,
                          reinterpret_cast<uint32_t*>(small + 2));
         int32_t* pin = static_cast<int32_t*>(ctx->pinned);
         XM_HIP(hipMemcpyAsync(pin, small, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -209,29 +212,31 @@ void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, 
         const uint32_t nbad = uint32_t(pin[2]);
         dfree(ctx, small);
         if (nbad) {
-            dfree(ctx, keys); dfree(ctx, keys_alt); dfree(ctx, vals); dfree(ctx, vals_alt);
+            dfree(ctx, words);
+            dfree(ctx, words_alt);
             throw InvalidArg{std::to_string(nbad) +
                              " record(s) with NaN or out-of-range coordinates "
                              "(need 0 <= ra < 360, -90 <= dec <= 90)"};
         }
         cat->zmin = zmin;
         cat->zmax = zmax;
-        // Digits of the key above the highest zone bit that differs between zmin
-        // and zmax are constant over the segment: those passes are skipped.
-        const int hi_bit = RA_BITS + bit_length(uint32_t(zmax));
+        // Sort on the zone and ra bits (from the digit holding the lowest ra bit; the
+        // row-index bits below it only order ties).  Zone digits whose bits agree
+        // between zmin and zmax are constant over the segment and skipped.
+        const int zone_lo = pl.ra_bits + pl.idx_bits;
+        const int hi_bit = zone_lo + bit_length(uint32_t(zmax));
         uint64_t skip = 0;
-        for (int d = RA_BITS / 8; d < 8; d++) {
-            const int zs = d * 8 - RA_BITS;  // first zone bit inside digit d (0..40)
+        for (int d = 0; d < 8; d++) {
+            if (d * 8 < zone_lo) continue;
+            const int zs = d * 8 - zone_lo;
             if ((uint64_t(uint32_t(zmin)) >> zs) == (uint64_t(uint32_t(zmax)) >> zs))
                 skip |= uint64_t(1) << d;
         }
-        radix_sort_pairs(ctx, &keys, &keys_alt, &vals, &vals_alt, n, SORT_LO_BIT, hi_bit, skip);
-        dfree(ctx, keys_alt);
+        radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit, skip);
+        dfree(ctx, words_alt);
         cat->row = dalloc(ctx, n * sizeof(Row64));
-        launch_gather(ctx, d_recs, vals, n, static_cast<Row64*>(cat->row));
-        dfree(ctx, vals);
-        dfree(ctx, vals_alt);
-        cat->key = keys;
+        launch_gather(ctx, d_recs, words, n, pl, static_cast<Row64*>(cat->row));
+        cat->key = words;
     }
     const int64_t ntab = int64_t(cat->zmax) - cat->zmin + 2;
     cat->zoff = static_cast<uint32_t*>(dalloc(ctx, size_t(ntab) * sizeof(uint32_t)));
@@ -328,6 +333,9 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
         if (n == "dense_min") {
             if (value < 0 || value > 1000000) throw InvalidArg{"dense_min out of range"};
             ctx->dense_min = uint32_t(value);
+        } else if (n == "key_ra_bits") {
+            if (value < 1 || value > RA_BITS) throw InvalidArg{"key_ra_bits must be in [1, 24]"};
+            ctx->key_ra_bits = int(value);
         } else {
             throw InvalidArg{"unknown parameter '" + n + "'"};
         }
@@ -340,6 +348,7 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
     return guarded(ctx, [&] {
         const std::string n = name;
         if (n == "dense_min") *value = ctx->dense_min;
+        else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
         else throw InvalidArg{"unknown parameter '" + n + "'"};
     });
 }
@@ -496,7 +505,7 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             throw InvalidArg{"precision mode not available in this build"};
         xm_catalog* A = const_cast<xm_catalog*>(a);
         ensure_chunks(ctx, A);
-        const PairParams pp = make_params(ctx, A, radius_deg);
+        const PairParams pp = make_params(ctx, A, b, radius_deg);
         uint64_t cap = capacity_hint ? capacity_hint : 2 * std::max(a->n, b->n);
         cap = std::max<uint64_t>(cap, 1024);
         XM_HIP(hipEventCreate(&ev0));

@@ -2,7 +2,7 @@
 // vectors, zone offsets, A-row chunks.
 //
 // Replaces Sorter::Sort (BoostXmatch/BoostXmatch/Sorter.cu:141-185):
-//   GetZoneId + less_zonera (Sorter.cu:20-43)   -> k_make_keys (packed sort keys)
+//   GetZoneId + less_zonera (Sorter.cu:20-43)   -> k_make_keys (packed sort words)
 //   lower_bound/upper_bound (Sorter.cu:161-164) -> k_zone_offsets
 //   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (one 32-byte row per source:
 //                                                  FP64 unit vector, radec2xyz of
@@ -25,9 +25,8 @@ __device__ inline bool valid_record(double ra, double dec) {
 // Grid-stride over the records with a capped grid, so the zone range and the
 // bad-record count cost one atomic per block rather than one per 256 records.
 __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ recs, uint32_t n,
-                                                   double h_deg, int32_t n_zones,
-                                                   uint64_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals,
+                                                   double h_deg, int32_t n_zones, PackLayout pl,
+                                                   uint64_t* __restrict__ words,
                                                    int32_t* __restrict__ zminmax,
                                                    uint32_t* __restrict__ bad) {
     __shared__ int32_t s_min[TPB / WAVE], s_max[TPB / WAVE];
@@ -38,17 +37,16 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
     for (uint64_t i = uint64_t(blockIdx.x) * TPB + threadIdx.x; i < n; i += stride) {
         const double ra = recs[i].ra_deg;
         const double dec = recs[i].dec_deg;
-        uint64_t key = ~uint64_t(0);
+        uint64_t word = ~uint64_t(0);
         if (valid_record(ra, dec)) {
             const int32_t z = zone_of(dec, h_deg, n_zones);
-            key = make_key(z, ra);
+            word = pack_word(pl, z, ra, i);
             zmn = min(zmn, z);
             zmx = max(zmx, z);
         } else {
             nbad++;
         }
-        keys[i] = key;
-        vals[i] = uint32_t(i);
+        words[i] = word;
     }
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) {
@@ -80,11 +78,14 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
 }
 
 __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
-                                                const uint32_t* __restrict__ perm, uint32_t n,
-                                                Row64* __restrict__ rows) {
+                                                uint64_t* __restrict__ words, uint32_t n,
+                                                PackLayout pl, Row64* __restrict__ rows) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = perm[i];
+    const uint64_t wd = words[i];
+    const uint64_t s = wd & ((uint64_t(1) << pl.idx_bits) - 1);
+    const uint64_t ra_q = (wd >> pl.idx_bits) & ((uint64_t(1) << pl.ra_bits) - 1);
+    const uint64_t zone = wd >> (pl.idx_bits + pl.ra_bits);
     const int64_t oid = recs[s].id;
     const double ra = recs[s].ra_deg;
     const double dec = recs[s].dec_deg;
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
     unit_vector(ra, dec, &r.x, &r.y, &r.z);
     r.id = oid;
     rows[i] = r;
+    words[i] = (zone << RA_BITS) | (ra_q << (RA_BITS - pl.ra_bits));
 }
 
 // zoff[z - zmin] = first sorted row whose zone is >= z, for z in [zmin, zmax+1].
@@ -144,20 +146,21 @@ inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 }  // namespace
 
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
-                      uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad) {
+                      PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad) {
     if (n == 0) return;
     const uint32_t nblk = std::min<uint32_t>(grid_for(n).x, 4096u);
     launch_timed(ctx, "make_keys", [&] {
         hipLaunchKernelGGL(k_make_keys, dim3(nblk), dim3(TPB), 0, ctx->stream, recs, uint32_t(n),
-                           h_deg, n_zones, keys, vals, zminmax, bad);
+                           h_deg, n_zones, pl, words, zminmax, bad);
     });
 }
 
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, Row64* rows) {
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n, PackLayout pl,
+                   Row64* rows) {
     if (n == 0) return;
     launch_timed(ctx, "gather", [&] {
-        hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, perm,
-                           uint32_t(n), rows);
+        hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, words,
+                           uint32_t(n), pl, rows);
     });
 }
 

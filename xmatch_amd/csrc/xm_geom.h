@@ -150,6 +150,47 @@ XM_HD int64_t alpha_quanta(double theta_deg, double h_deg, int32_t zone, double 
     return int64_t(q);
 }
 
+// Half-width of a join's window in ra quanta for A zone `zone`: alpha_quanta plus
+// the coarser of the two catalogs' ra granularities (gran quanta), since each
+// catalog's keys floor ra to its own granularity.  WHOLE_ZONE as alpha_quanta.
+XM_HD int64_t join_quanta(double theta_deg, double h_deg, int32_t zone, double atan_sin_theta,
+                          int64_t gran) {
+    int64_t aq = alpha_quanta(theta_deg, h_deg, zone, atan_sin_theta);
+    if (aq == WHOLE_ZONE) return WHOLE_ZONE;
+    aq += gran;
+    if (aq >= int64_t(uint64_t(1) << (RA_BITS - 1))) return WHOLE_ZONE;
+    return aq;
+}
+
+// Packed sort word of a catalog build: zone | ra (ra_bits) | input row (idx_bits),
+// 64 bits in all.  idx_bits = bits to hold the row index, zone_bits = bits to hold
+// the zone id, ra_bits = min(24, 64 - zone_bits - idx_bits); sorting the word on
+// its zone and ra bits orders the rows by (zone, ra) and carries each row's input
+// position along.  The catalog key keeps zone << 24 | ra << (24 - ra_bits).
+struct PackLayout {
+    int idx_bits, ra_bits, zone_bits;
+};
+
+XM_HD int bit_length_u64(uint64_t v) {
+    int b = 0;
+    while (v) { b++; v >>= 1; }
+    return b;
+}
+
+XM_HD PackLayout pack_layout(uint64_t n, int32_t n_zones) {
+    PackLayout p;
+    p.idx_bits = bit_length_u64(n > 0 ? n - 1 : 0);
+    p.zone_bits = bit_length_u64(uint64_t(n_zones > 0 ? n_zones - 1 : 0));
+    const int room = 64 - p.idx_bits - p.zone_bits;
+    p.ra_bits = room < RA_BITS ? room : RA_BITS;
+    return p;
+}
+
+XM_HD uint64_t pack_word(const PackLayout& p, int32_t zone, double ra_deg, uint64_t idx) {
+    const uint64_t q = ra_quantum(ra_deg) >> (RA_BITS - p.ra_bits);
+    return (uint64_t(uint32_t(zone)) << (p.ra_bits + p.idx_bits)) | (q << p.idx_bits) | idx;
+}
+
 // The (at most two) key intervals of B zone zb that a chunk whose A rows span ra
 // quanta [q_lo, q_hi] must test, for half-width aq (or WHOLE_ZONE).  Intervals are
 // inclusive key bounds [lo[i], hi[i]]; shift[i] maps the ra quanta of that interval

@@ -53,6 +53,7 @@ struct xm_ctx {
     void* pinned = nullptr;
     // tuning knobs (xm_ctx_set_param)
     uint32_t dense_min = 16;
+    int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
 };
 
 struct xm_catalog {
@@ -61,6 +62,7 @@ struct xm_catalog {
     double h_deg = 0;
     int32_t n_zones = 0;
     int32_t zmin = 0, zmax = -1;
+    int64_t ra_gran = 1;         // ra granularity of the keys, in 2^-24 turn quanta
     uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
     void* row = nullptr;         // xm::Row64[n]: FP64 unit vector + id, sorted order
     uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
@@ -119,17 +121,20 @@ inline void launch_timed(xm_ctx* ctx, const char* name, F&& f) {
 // ---- scan / sort (xm_sort.hip)
 // Exclusive prefix sum of n uint32 values in place.
 void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
-// LSD radix sort of (key, value) pairs on key bits [lo_bit, hi_bit); stable.
-// Buffers are ping-ponged; on return *keys / *vals point at the sorted data and
-// the other buffers hold scratch.
-void radix_sort_pairs(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint32_t** vals,
-                      uint32_t** vals_alt, uint64_t n, int lo_bit, int hi_bit,
-                      uint64_t skip_mask_digits);
+// Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), 8-bit digits,
+// skipping the digits flagged in skip_mask_digits.  Buffers are ping-ponged; on
+// return *keys points at the sorted keys and *keys_alt at scratch.
+void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
+                     int hi_bit, uint64_t skip_mask_digits);
 
 // ---- catalog kernels (xm_catalog.hip)
+// Packed sort words (pack_word) of the records, their zone range and bad count.
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
-                      uint64_t* keys, uint32_t* vals, int32_t* zminmax, uint32_t* bad);
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint32_t* perm, uint64_t n, Row64* rows);
+                      PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad);
+// From the sorted words: rows[i] (unit vector + id of the input row the word
+// carries) and, in place, words[i] -> catalog key zone << 24 | ra << (24 - ra_bits).
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n, PackLayout pl,
+                   Row64* rows);
 void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin,
                          int32_t zmax, uint32_t* zoff);
 void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts);
@@ -145,6 +150,7 @@ struct PairParams {
     double h_deg;           // zone height, degrees
     double atan_sin_theta;  // atan(sin(theta)), host-computed once
     double chord2;          // (2 sin(r/2))^2
+    int64_t gran;           // coarser ra granularity of the two catalogs (quanta)
     int32_t nz;             // ceil(r / h)
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()

@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256) void k_classify(const Chunk* __restrict__ chun
         ch = chunks[i];
         const uint64_t q_lo = akey[ch.start] & RA_MASK;
         const uint64_t q_hi = akey[ch.start + ch.count - 1] & RA_MASK;
-        const int64_t aq = alpha_quanta(pp.theta_deg, pp.h_deg, ch.zone, pp.atan_sin_theta);
+        const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, ch.zone, pp.atan_sin_theta,
+                                       pp.gran);
         dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
         ch.aq = int32_t(aq);  // WHOLE_ZONE (-1) only ever reaches the dense list
     }

@@ -1,9 +1,9 @@
 // xm_sort.hip -- device-wide exclusive scan and LSD radix sort of (key, index) pairs.
 //
-// Replaces thrust::sort(less_zonera) and the lower/upper_bound zone search of
-// BoostXmatch/BoostXmatch/Sorter.cu:153-164.  The keys are packed (zone << 40 |
-// ra quantum) words, so one stable LSD radix sort on the occupied bits orders the
-// segment by (zone, ra) without comparisons.
+// Replaces thrust::sort(less_zonera) of BoostXmatch/BoostXmatch/Sorter.cu:153.  The
+// keys are packed 64-bit words zone | ra | input row, so one stable LSD radix sort
+// on the zone and ra bits orders the segment by (zone, ra) without comparisons and
+// carries each row's input position in the low bits.
 //
 // Radix pass (8-bit digit), reduce-then-scan:
 //   k_radix_count   per 4096-key tile: digit histogram -> counts[digit][tile]
@@ -11,7 +11,7 @@
 //   k_radix_scatter per tile: wave64 ballot match of equal digits gives each key
 //                   its stable rank; keys are regrouped by digit in LDS, then
 //                   written out so equal digits land in contiguous runs.
-// Algorithmic HBM traffic per pass and key: 8 (count) + 12 (scatter read) + 12
+// Algorithmic HBM traffic per pass and key: 8 (count) + 8 (scatter read) + 8
 // (scatter write) bytes.
 #include "xm_internal.h"
 
@@ -176,11 +176,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_count(const uint64_t* __re
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ offs,
-    uint32_t nblocks) {
+    const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, uint32_t n, int shift,
+    const uint32_t* __restrict__ offs, uint32_t nblocks) {
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_v[RS_TILE];
     __shared__ uint32_t s_wc[4][256];   // per-wave running digit counts, then wave prefixes
     __shared__ uint32_t s_ts[256];      // tile-local start of each digit
     __shared__ uint32_t s_go[256];      // global start of each digit for this tile
@@ -197,14 +195,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t lt = (uint64_t(1) << lane) - 1u;
     const uint32_t base = blockIdx.x * RS_TILE + w * RS_WAVE_TILE;
     uint64_t key[RS_ITEMS];
-    uint32_t val[RS_ITEMS];
     uint32_t rk[RS_ITEMS];
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
+        key[k] = i < n ? kin[i] : ~uint64_t(0);
+    }
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const uint32_t i = base + k * WAVE + lane;
         const bool valid = i < n;
-        key[k] = valid ? kin[i] : ~uint64_t(0);
-        val[k] = valid ? vin[i] : 0u;
         const uint32_t d = uint32_t(key[k] >> shift) & 255u;
         // lanes holding the same digit: intersect the 8 per-bit ballots
         uint64_t m = __ballot(valid);
@@ -237,9 +237,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         const uint32_t i = base + k * WAVE + lane;
         if (i < n) {
             const uint32_t d = uint32_t(key[k] >> shift) & 255u;
-            const uint32_t lp = s_ts[d] + s_wc[w][d] + rk[k];
-            s_k[lp] = key[k];
-            s_v[lp] = val[k];
+            s_k[s_ts[d] + s_wc[w][d] + rk[k]] = key[k];
         }
     }
     __syncthreads();
@@ -248,9 +246,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     for (uint32_t p = tid; p < tile_n; p += RS_THREADS) {
         const uint64_t kk = s_k[p];
         const uint32_t d = uint32_t(kk >> shift) & 255u;
-        const uint32_t dst = s_go[d] + (p - s_ts[d]);
-        kout[dst] = kk;
-        vout[dst] = s_v[p];
+        kout[s_go[d] + (p - s_ts[d])] = kk;
     }
 }
 
@@ -283,9 +279,8 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
     dfree(ctx, partial);
 }
 
-void radix_sort_pairs(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint32_t** vals,
-                      uint32_t** vals_alt, uint64_t n, int lo_bit, int hi_bit,
-                      uint64_t skip_mask_digits) {
+void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
+                     int hi_bit, uint64_t skip_mask_digits) {
     if (n <= 1) return;
     if (n > 0xffffffffull) throw InvalidArg{"radix sort supports fewer than 2^32 keys"};
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
@@ -300,11 +295,10 @@ void radix_sort_pairs(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint32_
         scan_exclusive_u32(ctx, counts, uint64_t(256) * nblocks);
         launch_timed(ctx, "radix_scatter", [&] {
             hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
-                               (const uint64_t*)*keys, (const uint32_t*)*vals, *keys_alt, *vals_alt,
-                               uint32_t(n), shift, (const uint32_t*)counts, nblocks);
+                               (const uint64_t*)*keys, *keys_alt, uint32_t(n), shift,
+                               (const uint32_t*)counts, nblocks);
         });
         std::swap(*keys, *keys_alt);
-        std::swap(*vals, *vals_alt);
     }
     dfree(ctx, counts);
 }

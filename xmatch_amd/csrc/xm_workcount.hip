@@ -5,10 +5,11 @@
 //            the dense tests of the reference's zone loop (Worker.cu:330-359)
 //   w_cand = #{(a,b): |za-zb| <= nz, cyclic |ra_a-ra_b| <= alpha(za)}
 //            the pairs that reach the reference's chord test (Worker.cu:89-122),
-//            alpha = calc_alpha (Common.cu:16-23) without widening
+//            alpha = calc_alpha (Common.cu:16-23) without widening, ra = atan2(y, x)
+//            of the rows' unit vectors
 //   w_tile = sum over A chunks of rows(chunk) * B rows in the chunk's windows
 //            the tests the pair kernel executes (same integer window logic as
-//            the kernel: alpha_quanta + window_intervals)
+//            the kernel: join_quanta, window_intervals, chunk_is_dense)
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -49,18 +50,37 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
             return XM_EINVAL;
         }
         std::vector<uint64_t> ka(a->n), kb(b->n);
+        std::vector<Row64> ra_rows(a->n), rb_rows(b->n);
         const int64_t nta = int64_t(a->zmax) - a->zmin + 2, ntb = int64_t(b->zmax) - b->zmin + 2;
         std::vector<uint32_t> za(size_t(std::max<int64_t>(nta, 1))), zb(size_t(std::max<int64_t>(ntb, 1)));
-        if (a->n) XM_HIP(hipMemcpyAsync(ka.data(), a->key, a->n * 8, hipMemcpyDeviceToHost, ctx->stream));
-        if (b->n) XM_HIP(hipMemcpyAsync(kb.data(), b->key, b->n * 8, hipMemcpyDeviceToHost, ctx->stream));
-        XM_HIP(hipMemcpyAsync(za.data(), a->zoff, nta * 4, hipMemcpyDeviceToHost, ctx->stream));
-        XM_HIP(hipMemcpyAsync(zb.data(), b->zoff, ntb * 4, hipMemcpyDeviceToHost, ctx->stream));
+        auto d2h = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes) XM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        d2h(ka.data(), a->key, a->n * 8);
+        d2h(kb.data(), b->key, b->n * 8);
+        d2h(ra_rows.data(), a->row, a->n * sizeof(Row64));
+        d2h(rb_rows.data(), b->row, b->n * sizeof(Row64));
+        d2h(za.data(), a->zoff, nta * 4);
+        d2h(zb.data(), b->zoff, ntb * 4);
         XM_HIP(hipStreamSynchronize(ctx->stream));
+
+        // exact ra (radians in [0, 2pi)) of every row, from its unit vector
+        const double two_pi = 6.283185307179586476925286766559;
+        auto ra_of = [&](const std::vector<Row64>& rows) {
+            std::vector<double> r(rows.size());
+            for (size_t i = 0; i < rows.size(); i++) {
+                double v = std::atan2(rows[i].y, rows[i].x);
+                r[i] = v < 0 ? v + two_pi : v;
+            }
+            return r;
+        };
+        const std::vector<double> raA = ra_of(ra_rows), raB = ra_of(rb_rows);
 
         const double h = a->h_deg;
         const int32_t nz = int32_t(std::ceil(radius_deg / h));
         const int32_t n_zones = a->n_zones;
         const double atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
+        const int64_t gran = std::max(a->ra_gran, b->ra_gran);
         const int64_t full = int64_t(1) << RA_BITS;
 
         std::atomic<int64_t> next{a->n ? a->zmin : 1};
@@ -75,30 +95,30 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
                 const uint64_t a0 = za[z - a->zmin], a1 = za[z - a->zmin + 1];
                 if (a1 <= a0) continue;
                 const double al = ref_alpha_rad(radius_deg, h, int32_t(z));
-                const double al_q = al < 0 ? -1.0 : al * RAD2DEG * RA_SCALE;
-                const int64_t aq = alpha_quanta(radius_deg, h, int32_t(z), atan_sin_theta);
+                const int64_t aq = join_quanta(radius_deg, h, int32_t(z), atan_sin_theta, gran);
                 for (int64_t z2 = std::max<int64_t>(0, z - nz);
                      z2 <= std::min<int64_t>(n_zones - 1, z + nz); z2++) {
                     if (z2 < b->zmin || z2 > b->zmax) continue;
                     const uint64_t b0 = zb[z2 - b->zmin], b1 = zb[z2 - b->zmin + 1];
                     if (b1 <= b0) continue;
                     s_ref += (a1 - a0) * (b1 - b0);
-                    const uint64_t zk = uint64_t(z2) << RA_BITS;
-                    // w_cand: per A row, B rows within the unwidened alpha (cyclic)
+                    // w_cand: per A row, B rows within the unwidened alpha (cyclic, exact
+                    // ra); the key window of the row bounds the rows to look at
                     for (uint64_t i = a0; i < a1; i++) {
-                        if (al_q < 0 || 2 * al_q + 1 >= double(full)) { s_cand += b1 - b0; continue; }
-                        const double q = double(ka[i] & RA_MASK);
-                        const double lo = q - al_q, hi = q + al_q;
-                        auto cnt = [&](double l, double r) -> uint64_t {  // keys with l <= q <= r
-                            const uint64_t li = uint64_t(std::max(0.0, std::ceil(l)));
-                            const double rr = std::floor(r);
-                            if (rr < 0 || double(li) > rr) return 0;
-                            const uint64_t ri = uint64_t(std::min(rr, double(RA_MASK)));
-                            return lower(kb, b0, b1, zk + ri + 1) - lower(kb, b0, b1, zk + li);
-                        };
-                        if (lo < 0) s_cand += cnt(lo + full, double(RA_MASK)) + cnt(0, hi);
-                        else if (hi > double(RA_MASK)) s_cand += cnt(lo, double(RA_MASK)) + cnt(0, hi - full);
-                        else s_cand += cnt(lo, hi);
+                        if (al < 0 || aq == WHOLE_ZONE) { s_cand += b1 - b0; continue; }
+                        const uint64_t q = ka[i] & RA_MASK;
+                        uint64_t lo[2], hi[2];
+                        int32_t shift[2];
+                        const int ni = window_intervals(int32_t(z2), q, q, aq, lo, hi, shift);
+                        for (int t2 = 0; t2 < ni; t2++) {
+                            const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
+                            const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                            for (uint64_t j = j0; j < j1; j++) {
+                                double d = std::fabs(raA[i] - raB[j]);
+                                if (d > two_pi - d) d = two_pi - d;
+                                if (d <= al) s_cand++;
+                            }
+                        }
                     }
                     // w_tile: the kernel's chunks, windows and per-chunk mode
                     for (uint64_t s = a0; s < a1; s += CHUNK_ROWS) {
@@ -140,6 +160,7 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
         if (w_ref) *w_ref = s1;
         if (w_cand) *w_cand = s2;
         if (w_tile) *w_tile = s3;
+        (void)full;
         return XM_OK;
     } catch (const DeviceError& e) {
         ctx->last_error = e.what;
