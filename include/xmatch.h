@@ -79,6 +79,9 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  *                against every chunk row) when a B row is expected to meet at
  *                least this many chunk rows, else sparsely (each B row against the
  *                chunk rows within the ra window).  0 = always dense.  Default 16.
+ *   "key_ra_bits" upper bound (1..24) on the ra bits kept in a catalog's sort key
+ *                (applied at catalog creation; large catalogs keep fewer, so that
+ *                zone | ra | row index fits 64 bits).  Default 24.
  * Results never depend on the knobs; only the work done does. */
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
