@@ -40,7 +40,7 @@ BYTES_PER_SOURCE = {
 }
 KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
            "scan_apply", "gather", "zone_offsets", "chunk_counts", "chunk_fill", "fill",
-           "classify", "pair_sparse", "pair_dense", "compact"]
+           "pair_fp64", "region_scan", "compact"]
 
 
 def log(msg):
@@ -136,7 +136,7 @@ def main():
 
     # dominant kernel by total device time
     dom = max(ktimes.items(), key=lambda kv: kv[1]["total_ms"])[0]
-    if dom in ("pair_sparse", "pair_dense"):
+    if dom == "pair_fp64":
         # 9 FP64 VALU operations per executed pair test (3 sub, 3 mul, 2 add, 1 cmp)
         flops = w["w_tile"] * 9.0
         ach = flops / (ktimes[dom]["avg_ms"] / 1e3) / 1e12

@@ -59,7 +59,6 @@ typedef struct {
     uint64_t n_retries;      /* result-buffer overflows that forced a re-run */
     uint64_t band_hits;      /* FP32 filter only: pairs inside the guard band (FP64 rechecked) */
     double join_ms;          /* device time of the join (HIP events), including re-runs */
-    uint64_t n_dense_chunks; /* chunks joined in dense mode (the rest sparse) */
 } xm_match_stats;
 
 /* ---- context: replaces CudaManager/CudaContext (CudaManager.cpp:8-65,
@@ -87,8 +86,12 @@ xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
 /* Wait for all device work of the context. */
 xm_status xm_ctx_synchronize(xm_ctx *ctx);
-/* Device malloc/free through the context's stream-ordered pool (for callers that
- * stage records on the device). */
+/* The context keeps the device memory of finished catalogs, results and scratch
+ * for reuse.  This returns every cached block that is not in use to the device;
+ * cached_bytes (may be NULL) receives what the context still holds. */
+xm_status xm_ctx_release_memory(xm_ctx *ctx, uint64_t *cached_bytes);
+/* Device malloc/free through the context's memory cache (for callers that stage
+ * records on the device). */
 xm_status xm_device_alloc(xm_ctx *ctx, size_t bytes, void **dptr);
 xm_status xm_device_free(xm_ctx *ctx, void *dptr);
 xm_status xm_memcpy_h2d(xm_ctx *ctx, void *dst, const void *src, size_t bytes);

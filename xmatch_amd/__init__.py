@@ -31,7 +31,7 @@ _STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_E
 EXPORTED = [
     "xm_abi_version", "xm_ctx_create", "xm_ctx_destroy", "xm_last_error_message",
     "xm_ctx_set_timing", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
-    "xm_ctx_set_param", "xm_ctx_get_param",
+    "xm_ctx_set_param", "xm_ctx_get_param", "xm_ctx_release_memory",
     "xm_device_alloc", "xm_device_free", "xm_memcpy_h2d", "xm_memcpy_d2h",
     "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
     "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
@@ -48,7 +48,7 @@ class XmatchError(RuntimeError):
 class MatchStats(ctypes.Structure):
     _fields_ = [("n_matches", ctypes.c_uint64), ("n_chunks", ctypes.c_uint64),
                 ("n_retries", ctypes.c_uint64), ("band_hits", ctypes.c_uint64),
-                ("join_ms", ctypes.c_double), ("n_dense_chunks", ctypes.c_uint64)]
+                ("join_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,6 +77,7 @@ def lib():
                                     ctypes.POINTER(u64)], ctypes.c_int),
             "xm_ctx_reset_timing": ([vp], ctypes.c_int),
             "xm_ctx_synchronize": ([vp], ctypes.c_int),
+            "xm_ctx_release_memory": ([vp, ctypes.POINTER(u64)], ctypes.c_int),
             "xm_ctx_set_param": ([vp, ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
             "xm_ctx_get_param": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
                                  ctypes.c_int),
@@ -157,6 +158,12 @@ class Context:
 
     def synchronize(self):
         self._check(lib().xm_ctx_synchronize(self._h))
+
+    def release_memory(self):
+        """Return cached device blocks; returns the bytes the context still holds."""
+        b = ctypes.c_uint64()
+        self._check(lib().xm_ctx_release_memory(self._h, ctypes.byref(b)))
+        return b.value
 
     def set_param(self, name, value):
         self._check(lib().xm_ctx_set_param(self._h, name.encode(), int(value)))

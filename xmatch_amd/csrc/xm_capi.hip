@@ -62,7 +62,8 @@ int bit_length(uint32_t v) {
 }
 
 // Build the A-side chunk table: zones in [zmin, zmax] split into runs of at most
-// CHUNK_ROWS rows.  One synchronisation (the chunk count sizes the table).
+// CHUNK_ROWS rows.  No synchronisation: the table is sized for the upper bound
+// n / CHUNK_ROWS + zones and the kernels read the true count from the device.
 void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     if (A->chunks || A->n == 0) return;
     const int32_t nzone = A->zmax - A->zmin + 1;
@@ -70,15 +71,13 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     launch_fill_u32(ctx, counts + nzone, 1, 0u);
     launch_chunk_counts(ctx, A->zoff, nzone, counts);
     scan_exclusive_u32(ctx, counts, uint64_t(nzone) + 1);
-    uint32_t* pin = static_cast<uint32_t*>(ctx->pinned);
-    XM_HIP(hipMemcpyAsync(pin, counts + nzone, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
-    const uint64_t total = pin[0];
-    Chunk* chunks = static_cast<Chunk*>(dalloc(ctx, std::max<uint64_t>(total, 1) * sizeof(Chunk)));
+    const uint64_t bound = A->n / CHUNK_ROWS + uint64_t(nzone);
+    Chunk* chunks = static_cast<Chunk*>(dalloc(ctx, std::max<uint64_t>(bound, 1) * sizeof(Chunk)));
     launch_chunk_fill(ctx, A->zoff, counts, A->zmin, nzone, chunks);
-    dfree(ctx, counts);
     A->chunks = chunks;
-    A->n_chunks = total;
+    A->max_chunks = bound;
+    A->chunk_start = counts;
+    A->n_chunks_dev = counts + nzone;
 }
 
 PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
@@ -109,20 +108,69 @@ void check_join_args(const xm_catalog* a, const xm_catalog* b, double radius_deg
 
 namespace xm {
 
+namespace {
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    size_t p = 256;
+    while (p * 2 < bytes) p *= 2;  // p < bytes <= 2p
+    for (int q = 1; q <= 4; q++) {
+        const size_t c = p + (p / 4) * size_t(q);
+        if (c >= bytes) return c;
+    }
+    return 2 * p;
+}
+}  // namespace
+
 void* dalloc(xm_ctx* ctx, size_t bytes) {
+    const size_t cls = size_class(bytes ? bytes : 1);
+    for (size_t i = ctx->free_blocks.size(); i-- > 0;) {
+        if (ctx->free_blocks[i].first == cls) {
+            void* p = ctx->free_blocks[i].second;
+            ctx->free_blocks[i] = ctx->free_blocks.back();
+            ctx->free_blocks.pop_back();
+            ctx->live_blocks.push_back({p, cls});
+            return p;
+        }
+    }
     void* p = nullptr;
-    if (bytes == 0) bytes = 8;
-    hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
+    hipError_t e = hipMalloc(&p, cls);
+    if (e == hipErrorOutOfMemory && !ctx->free_blocks.empty()) {
+        // give the cache back (after the stream drains) and try once more
+        (void)hipGetLastError();
+        release_cache(ctx);
+        e = hipMalloc(&p, cls);
+    }
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
         throw OutOfMemory{"device allocation of " + std::to_string(bytes) + " bytes failed"};
     }
     XM_HIP(e);
+    ctx->cached_bytes += cls;
+    ctx->live_blocks.push_back({p, cls});
     return p;
 }
 
 void dfree(xm_ctx* ctx, void* p) {
-    if (p) XM_HIP(hipFreeAsync(p, ctx->stream));
+    if (!p) return;
+    for (size_t i = ctx->live_blocks.size(); i-- > 0;) {
+        if (ctx->live_blocks[i].first == p) {
+            ctx->free_blocks.push_back({ctx->live_blocks[i].second, p});
+            ctx->live_blocks[i] = ctx->live_blocks.back();
+            ctx->live_blocks.pop_back();
+            return;
+        }
+    }
+    throw std::runtime_error("dfree of a pointer the context did not allocate");
+}
+
+void release_cache(xm_ctx* ctx) {
+    if (ctx->free_blocks.empty()) return;
+    XM_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& fb : ctx->free_blocks) {
+        XM_HIP(hipFree(fb.second));
+        ctx->cached_bytes -= fb.first;
+    }
+    ctx->free_blocks.clear();
 }
 
 void timing_begin(xm_ctx* ctx, const char* name, hipEvent_t* ev) {
@@ -183,7 +231,9 @@ void catalog_free(xm_catalog* c) {
     dfree(ctx, c->row);
     dfree(ctx, c->zoff);
     dfree(ctx, c->chunks);
+    dfree(ctx, c->chunk_start);
     c->key = nullptr; c->row = nullptr; c->zoff = nullptr; c->chunks = nullptr;
+    c->chunk_start = nullptr; c->n_chunks_dev = nullptr;
 }
 
 void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, xm_catalog* cat) {
@@ -263,10 +313,6 @@ xm_status xm_ctx_create(int device, xm_ctx** out) {
                              std::to_string(ndev) + " visible)"};
         XM_HIP(hipSetDevice(device));
         XM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        hipMemPool_t pool;
-        XM_HIP(hipDeviceGetDefaultMemPool(&pool, device));
-        uint64_t thr = UINT64_MAX;
-        XM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
         XM_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
     });
     if (st != XM_OK) {
@@ -287,6 +333,8 @@ void xm_ctx_destroy(xm_ctx* ctx) {
         (void)hipEventDestroy(l.stop);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto& fb : ctx->free_blocks) (void)hipFree(fb.second);
+    for (auto& lb : ctx->live_blocks) (void)hipFree(lb.first);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -350,6 +398,15 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
         if (n == "dense_min") *value = ctx->dense_min;
         else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
         else throw InvalidArg{"unknown parameter '" + n + "'"};
+    });
+}
+
+xm_status xm_ctx_release_memory(xm_ctx* ctx, uint64_t* cached_bytes) {
+    if (!ctx) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        release_cache(ctx);
+        if (cached_bytes) *cached_bytes = ctx->cached_bytes;
     });
 }
 
@@ -493,9 +550,8 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     xm_result* res = new (std::nothrow) xm_result();
     if (!res) return XM_ENOMEM;
     res->ctx = ctx;
-    uint64_t* counters = nullptr;
-    Chunk* list = nullptr;
-    uint32_t ns = 0, nd = 0;
+    RegionOut ro{};
+    uint64_t* scratch = nullptr;  // prefix[n_regions] + tail[2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     xm_match_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
@@ -510,93 +566,64 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         cap = std::max<uint64_t>(cap, 1024);
         XM_HIP(hipEventCreate(&ev0));
         XM_HIP(hipEventCreate(&ev1));
-        // split the chunks into sparse and dense lists for this radius
-        ns = 0;
-        nd = 0;
-        if (A->n_chunks) {
-            list = static_cast<Chunk*>(dalloc(ctx, A->n_chunks * sizeof(Chunk)));
-            uint32_t* cnts = static_cast<uint32_t*>(dalloc(ctx, 2 * sizeof(uint32_t)));
-            launch_fill_u32(ctx, cnts, 2, 0u);
-            XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_classify(ctx, A, pp, list, cnts);
-            XM_HIP(hipEventRecord(ev1, ctx->stream));
-            uint32_t* pin = static_cast<uint32_t*>(ctx->pinned);
-            XM_HIP(hipMemcpyAsync(pin, cnts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            sync(ctx);
-            ns = pin[0];
-            nd = pin[1];
-            dfree(ctx, cnts);
-            float ms = 0.f;
-            XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-            st_local.join_ms += ms;
-        }
-        // hits go to n_regions append regions (one per group of blocks), then are
-        // packed into the contiguous result
-        const uint32_t nblk = std::max(pair_blocks(ns), pair_blocks(nd));
-        RegionOut ro;
-        ro.n_regions = std::max<uint32_t>(1, std::min<uint32_t>(MAX_REGIONS, nblk));
+        // Hits go to n_regions append regions (blocks spread over them round-robin),
+        // a one-block scan sizes them and k_compact packs them into the result; the
+        // only host round trip reads the total and the fullest region.
+        ro.n_regions = std::max<uint32_t>(
+            1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(uint32_t(std::max<uint64_t>(A->max_chunks, 1)))));
         ro.region_cap = std::max<uint64_t>(1, (cap + ro.n_regions - 1) / ro.n_regions);
         ro.counters = static_cast<uint64_t*>(
             dalloc(ctx, size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t)));
-        counters = ro.counters;
+        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(ro.n_regions) + 2) * sizeof(uint64_t)));
+        uint64_t* prefix = scratch;
+        uint64_t* tail = scratch + ro.n_regions;
         uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
         for (int attempt = 0;; attempt++) {
-            ro.id_a = static_cast<int64_t*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
-            ro.id_b = static_cast<int64_t*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
-            ro.d2 = static_cast<double*>(dalloc(ctx, ro.n_regions * ro.region_cap * 8));
+            const uint64_t slots = uint64_t(ro.n_regions) * ro.region_cap;
+            ro.id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+            ro.id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+            ro.d2 = static_cast<double*>(dalloc(ctx, slots * 8));
+            res->id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+            res->id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+            res->dist = static_cast<double*>(dalloc(ctx, slots * 8));
+            res->cap = slots;
             launch_fill_u64(ctx, ro.counters, size_t(ro.n_regions) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_pair_fp64(ctx, A, b, pp, list, ns, nd, ro);
+            launch_pair_fp64(ctx, A, b, pp, ro);
+            launch_region_scan(ctx, ro, prefix, tail);
+            launch_compact(ctx, ro, prefix, res->id_a, res->id_b, res->dist);
             XM_HIP(hipEventRecord(ev1, ctx->stream));
-            XM_HIP(hipMemcpyAsync(pin, ro.counters,
-                                  size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t),
-                                  hipMemcpyDeviceToHost, ctx->stream));
+            XM_HIP(hipMemcpyAsync(pin, tail, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            if (A->n_chunks_dev)
+                XM_HIP(hipMemcpyAsync(pin + 2, A->n_chunks_dev, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
+            ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr;
             sync(ctx);
             float ms = 0.f;
             XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
             st_local.join_ms += ms;
-            uint64_t total = 0, most = 0;
-            uint64_t* prefix_h = pin + size_t(ro.n_regions) * REGION_STRIDE;
-            for (uint32_t r = 0; r < ro.n_regions; r++) {
-                const uint64_t c = pin[size_t(r) * REGION_STRIDE];
-                prefix_h[r] = total;
-                total += c;
-                most = std::max(most, c);
-            }
+            const uint64_t total = pin[0], most = pin[1];
+            st_local.n_chunks = A->n_chunks_dev ? reinterpret_cast<uint32_t*>(pin + 2)[0] : 0;
             if (most <= ro.region_cap) {
                 res->n = total;
-                res->cap = total;
-                res->id_a = static_cast<int64_t*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
-                res->id_b = static_cast<int64_t*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
-                res->dist = static_cast<double*>(dalloc(ctx, std::max<uint64_t>(total, 1) * 8));
-                uint64_t* prefix_d = static_cast<uint64_t*>(dalloc(ctx, ro.n_regions * 8));
-                XM_HIP(hipMemcpyAsync(prefix_d, prefix_h, ro.n_regions * 8, hipMemcpyHostToDevice,
-                                      ctx->stream));
-                XM_HIP(hipEventRecord(ev0, ctx->stream));
-                launch_compact(ctx, ro, prefix_d, res->id_a, res->id_b, res->dist);
-                XM_HIP(hipEventRecord(ev1, ctx->stream));
-                dfree(ctx, prefix_d);
-                dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
-                XM_HIP(hipEventSynchronize(ev1));
-                float ms2 = 0.f;
-                XM_HIP(hipEventElapsedTime(&ms2, ev0, ev1));
-                st_local.join_ms += ms2;
                 break;
             }
             // overflow: every counter holds its region's exact total; re-run with
             // regions that fit the fullest one
             st_local.n_retries++;
-            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
+            dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist);
+            res->id_a = nullptr; res->id_b = nullptr; res->dist = nullptr;
             ro.region_cap = most;
             if (attempt > 2) throw std::runtime_error("result size did not converge");
         }
         st_local.n_matches = res->n;
-        st_local.n_chunks = A->n_chunks;
-        st_local.n_dense_chunks = nd;
     });
     (void)guarded(ctx, [&] {
-        if (counters) dfree(ctx, counters);
-        if (list) dfree(ctx, list);
+        dfree(ctx, ro.counters);
+        dfree(ctx, scratch);
+        dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
         if (ev0) XM_HIP(hipEventDestroy(ev0));
         if (ev1) XM_HIP(hipEventDestroy(ev1));
     });

@@ -51,6 +51,10 @@ struct xm_ctx {
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
     // pinned host scratch for small device->host reads
     void* pinned = nullptr;
+    // device memory cache: blocks keyed by size class; reused in stream order
+    std::vector<std::pair<size_t, void*>> free_blocks;          // (class bytes, ptr)
+    std::vector<std::pair<void*, size_t>> live_blocks;          // ptr -> class bytes
+    size_t cached_bytes = 0;
     // tuning knobs (xm_ctx_set_param)
     uint32_t dense_min = 16;
     int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
@@ -66,9 +70,12 @@ struct xm_catalog {
     uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
     void* row = nullptr;         // xm::Row64[n]: FP64 unit vector + id, sorted order
     uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
-    // A-side row chunks (built on first use as the A catalog of a join)
+    // A-side row chunks (built on first use as the A catalog of a join): the
+    // table is sized for max_chunks, the true count stays on the device
     void* chunks = nullptr;
-    uint64_t n_chunks = 0;
+    uint64_t max_chunks = 0;
+    uint32_t* chunk_start = nullptr;   // [zmax-zmin+2]: exclusive scan of chunks per zone
+    const uint32_t* n_chunks_dev = nullptr;  // = chunk_start + (zmax-zmin+1)
 };
 
 struct xm_result {
@@ -101,9 +108,14 @@ struct Chunk {
     int32_t aq;
 };
 
-// ---- memory (stream-ordered pool on the context stream)
+// ---- memory: the context's cache of device blocks.  Every kernel of a context
+// runs on its one stream, so a block freed after its last use is enqueued can be
+// handed to later work on that stream at once.  Sizes are rounded up to a class
+// (4 classes per power of two, at most 25% slack); hipMalloc is called only when
+// no cached block of the class is free.  xm_ctx_release_memory returns the cache.
 void* dalloc(xm_ctx* ctx, size_t bytes);
 void dfree(xm_ctx* ctx, void* p);
+void release_cache(xm_ctx* ctx);
 
 // ---- launch bookkeeping: events around a kernel when timing is on
 void timing_begin(xm_ctx* ctx, const char* name, hipEvent_t* ev);
@@ -155,10 +167,6 @@ struct PairParams {
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
 };
-// Splits A's chunks into sparse ones (list[0, ns)) and dense ones (list[n-nd, n)),
-// filling each entry's aq; counts[0] = ns, counts[1] = nd (device, pre-zeroed).
-void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
-                     uint32_t* counts);
 // Hits are appended to n_regions regions of region_cap slots; block b appends to
 // region b % n_regions through counters[region * REGION_STRIDE] (one 128-byte line
 // per counter, so the waves of a join never queue on one address).  Counters count
@@ -175,8 +183,10 @@ struct RegionOut {
 };
 uint32_t pair_blocks(uint32_t n_chunks);
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
-                      const PairParams& pp, const Chunk* list, uint32_t ns, uint32_t nd,
-                      const RegionOut& ro);
+                      const PairParams& pp, const RegionOut& ro);
+// prefix[r] = hits of the regions before r (capped at region_cap each);
+// tail[0] = total hits, tail[1] = largest region count.
+void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail);
 // Packs the regions into contiguous arrays (region r's hits at prefix[r]) and
 // converts chord^2 to the FP64 distance in arcsec.
 void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,

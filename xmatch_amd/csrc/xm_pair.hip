@@ -4,8 +4,8 @@
 // (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
 // into chunks of up to 64 consecutive rows of one zone; each wavefront owns one
 // chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
-// alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  A
-// classify pass splits the chunks of a join into two lists:
+// alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  Each
+// chunk is joined in one of two modes, chosen per chunk (chunk_is_dense):
 //
 //  sparse  the chunk's A unit vectors and ra quanta are staged in LDS; each lane
 //          takes one B row, checks it against the window, binary-searches the
@@ -177,40 +177,6 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ co
     }
 }
 
-// ------------------------------------------------------------- classify ----
-__global__ __launch_bounds__(256) void k_classify(const Chunk* __restrict__ chunks, uint32_t n,
-                                                  const uint64_t* __restrict__ akey, PairParams pp,
-                                                  Chunk* __restrict__ list,
-                                                  uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t lane = lane_id();
-    const bool valid = i < n;
-    Chunk ch{0, 0, 0, 0};
-    bool dense = false;
-    if (valid) {
-        ch = chunks[i];
-        const uint64_t q_lo = akey[ch.start] & RA_MASK;
-        const uint64_t q_hi = akey[ch.start + ch.count - 1] & RA_MASK;
-        const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, ch.zone, pp.atan_sin_theta,
-                                       pp.gran);
-        dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
-        ch.aq = int32_t(aq);  // WHOLE_ZONE (-1) only ever reaches the dense list
-    }
-    const uint64_t md = __ballot(valid && dense);
-    const uint64_t ms = __ballot(valid && !dense);
-    uint32_t bd = 0, bs = 0;
-    if (lane == 0) {
-        if (md) bd = atomicAdd(&counts[1], uint32_t(__popcll(md)));
-        if (ms) bs = atomicAdd(&counts[0], uint32_t(__popcll(ms)));
-    }
-    bd = __shfl(bd, 0, WAVE);
-    bs = __shfl(bs, 0, WAVE);
-    if (valid) {
-        if (dense) list[n - 1 - (bd + mbcnt64(md))] = ch;
-        else list[bs + mbcnt64(ms)] = ch;
-    }
-}
-
 // --------------------------------------------------------------- sparse ----
 struct SparseCtx {
     uint32_t start, cnt;
@@ -268,11 +234,44 @@ __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, ui
     }
 }
 
-__global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
-    const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
-    const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
-    const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
-    int32_t bzmax, PairParams pp, RegionOut ro) {
+// Dense mode: B rows [j0, j1) against the lane's A row (registers), the rows
+// arriving as wide scalar loads from a uniform base, one branch per UNROLL masks.
+__device__ inline void dense_range(uint32_t j0, uint32_t j1, double ax, double ay, double az,
+                                   uint32_t row, double thr, const Row64* __restrict__ brow,
+                                   uint32_t lane, Stage& st, const Out& o) {
+    uint32_t j = j0;
+    for (; j + UNROLL <= j1; j += UNROLL) {
+        const Row64* __restrict__ pb = brow + size_t(j);
+        double d2[UNROLL];
+        uint64_t m[UNROLL];
+        uint64_t any = 0;
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            d2[u] = chord2(ax, ay, az, pb[u].x, pb[u].y, pb[u].z);
+            m[u] = __ballot(d2[u] < thr);
+            any |= m[u];
+        }
+        if (any) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++)
+                if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, st, o);
+        }
+    }
+    for (; j < j1; j++) {
+        const double d2 = chord2(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
+        const uint64_t m = __ballot(d2 < thr);
+        if (m) stage_hits(m, d2, row, j, lane, st, o);
+    }
+}
+
+// One wavefront per A chunk; the chunk count lives on the device (*n_chunks), the
+// grid covers an upper bound and the waves past the count return at once.
+__global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
+    const Chunk* __restrict__ chunks, const uint32_t* __restrict__ n_chunks,
+    const uint64_t* __restrict__ akey, const Row64* __restrict__ arow,
+    const uint64_t* __restrict__ bkey, const Row64* __restrict__ brow,
+    const uint32_t* __restrict__ bzoff, int32_t bzmin, int32_t bzmax, PairParams pp,
+    RegionOut ro) {
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
@@ -284,31 +283,35 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
     const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
-    if (c >= n_list) return;  // whole wavefront; the kernel has no block barrier
+    if (c >= *n_chunks) return;  // whole wavefront; the kernel has no block barrier
 
-    const uint32_t start = uni(list[c].start);
-    const uint32_t cnt = uni(list[c].count);
-    const int32_t za = int32_t(uni(uint32_t(list[c].zone)));
-    const int32_t aq = int32_t(uni(uint32_t(list[c].aq)));
+    const uint32_t start = uni(chunks[c].start);
+    const uint32_t cnt = uni(chunks[c].count);
+    const int32_t za = int32_t(uni(uint32_t(chunks[c].zone)));
 
+    const uint32_t row = start + lane;
     int32_t qa = INT_MAX;
-    double ax = 10.0, ay = 10.0, az = 10.0;
+    double ax = 10.0, ay = 10.0, az = 10.0;  // inactive lanes sit far off the sphere
     if (lane < cnt) {
-        const Row64 a = arow[start + lane];
+        const Row64 a = arow[row];
         ax = a.x;
         ay = a.y;
         az = a.z;
-        qa = int32_t(akey[start + lane] & RA_MASK);
+        qa = int32_t(akey[row] & RA_MASK);
     }
-    s_aq[w][lane] = qa;
-    s_ax[w][lane] = ax;
-    s_ay[w][lane] = ay;
-    s_az[w][lane] = az;
-    wave_lds_sync();
     const uint64_t q_lo = uint32_t(__builtin_amdgcn_readlane(qa, 0));
     const uint64_t q_hi = uint32_t(__builtin_amdgcn_readlane(qa, int(cnt - 1)));
+    const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, za, pp.atan_sin_theta, pp.gran);
+    const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
+    if (!dense) {
+        s_aq[w][lane] = qa;
+        s_ax[w][lane] = ax;
+        s_ay[w][lane] = ay;
+        s_az[w][lane] = az;
+        wave_lds_sync();
+    }
 
-    const SparseCtx x{start, cnt, aq, s_aq[w], s_ax[w], s_ay[w], s_az[w], pp.chord2};
+    const SparseCtx x{start, cnt, int32_t(aq), s_aq[w], s_ax[w], s_ay[w], s_az[w], pp.chord2};
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
     const Out o = make_out(arow, brow, ro);
 
@@ -321,6 +324,14 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
         uint64_t lo[2], hi[2];
         int32_t shift[2], lq[2], hq[2];
         const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
+        if (dense) {
+            for (int t = 0; t < ni; t++) {
+                const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
+                const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
+                dense_range(j0, j1, ax, ay, az, row, pp.chord2, brow, lane, st, o);
+            }
+            continue;
+        }
         for (int t = 0; t < ni; t++) {
             lq[t] = int32_t(lo[t] & RA_MASK);
             hq[t] = int32_t(hi[t] & RA_MASK);
@@ -355,81 +366,35 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_sparse(
     if (st.n) flush_stage(st, lane, o);
 }
 
-// ---------------------------------------------------------------- dense ----
-__global__ __launch_bounds__(PK_THREADS) void k_pair_dense(
-    const Chunk* __restrict__ list, uint32_t n_list, const uint64_t* __restrict__ akey,
-    const Row64* __restrict__ arow, const uint64_t* __restrict__ bkey,
-    const Row64* __restrict__ brow, const uint32_t* __restrict__ bzoff, int32_t bzmin,
-    int32_t bzmax, PairParams pp, RegionOut ro) {
-    __shared__ uint32_t s_ra[PK_WAVES][STAGE];
-    __shared__ uint32_t s_rb[PK_WAVES][STAGE];
-    __shared__ double s_d2[PK_WAVES][STAGE];
-
-    const uint32_t lane = lane_id();
-    const uint32_t w = uni(threadIdx.x / WAVE);
-    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
-    if (c >= n_list) return;  // whole wavefront; the kernel has no block barrier
-
-    const uint32_t start = uni(list[c].start);
-    const uint32_t cnt = uni(list[c].count);
-    const int32_t za = int32_t(uni(uint32_t(list[c].zone)));
-    const int64_t aq = int64_t(int32_t(uni(uint32_t(list[c].aq))));
-
-    const uint32_t row = start + lane;
-    double ax = 10.0, ay = 10.0, az = 10.0;  // inactive lanes sit far off the sphere
-    if (lane < cnt) {
-        const Row64 a = arow[row];
-        ax = a.x;
-        ay = a.y;
-        az = a.z;
+// prefix[r] = sum of the capped counts of regions < r; tail[0] = total hits,
+// tail[1] = largest region count (uncapped).  One block.
+__global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict__ counters,
+                                                      uint32_t n_regions, uint64_t region_cap,
+                                                      uint64_t* __restrict__ prefix,
+                                                      uint64_t* __restrict__ tail) {
+    __shared__ uint64_t s_sum[1024];
+    __shared__ uint64_t s_max[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t c = t < n_regions ? counters[size_t(t) * REGION_STRIDE] : 0;
+    s_sum[t] = c < region_cap ? c : region_cap;
+    s_max[t] = c;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the capped counts; max by tree reduction
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint64_t v = t >= d ? s_sum[t - d] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
     }
-    const uint64_t q_lo = akey[start] & RA_MASK;
-    const uint64_t q_hi = akey[start + cnt - 1] & RA_MASK;
-    const double thr = pp.chord2;
-
-    Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o = make_out(arow, brow, ro);
-
-    for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
-        const int32_t zb = za + dz;
-        if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
-        const uint32_t b0 = uni(bzoff[zb - bzmin]);
-        const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
-        if (b0 >= b1) continue;
-        uint64_t lo[2], hi[2];
-        int32_t shift[2];
-        const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
-        for (int t = 0; t < ni; t++) {
-            const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
-            const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
-            uint32_t j = j0;
-            // UNROLL B rows per step: the rows arrive as wide scalar loads from a
-            // uniform base, and one branch covers the UNROLL hit masks
-            for (; j + UNROLL <= j1; j += UNROLL) {
-                const Row64* __restrict__ pb = brow + size_t(j);
-                double d2[UNROLL];
-                uint64_t m[UNROLL];
-                uint64_t any = 0;
-#pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    d2[u] = chord2(ax, ay, az, pb[u].x, pb[u].y, pb[u].z);
-                    m[u] = __ballot(d2[u] < thr);
-                    any |= m[u];
-                }
-                if (any) {
-#pragma unroll
-                    for (int u = 0; u < UNROLL; u++)
-                        if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, st, o);
-                }
-            }
-            for (; j < j1; j++) {
-                const double d2 = chord2(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
-                const uint64_t m = __ballot(d2 < thr);
-                if (m) stage_hits(m, d2, row, j, lane, st, o);
-            }
-        }
+    for (uint32_t d = 512; d >= 1; d >>= 1) {
+        if (t < d && s_max[t + d] > s_max[t]) s_max[t] = s_max[t + d];
+        __syncthreads();
     }
-    if (st.n) flush_stage(st, lane, o);
+    if (t < n_regions) prefix[t] = s_sum[t] - (c < region_cap ? c : region_cap);
+    if (t == 0) {
+        tail[0] = s_sum[1023];
+        tail[1] = s_max[0];
+    }
 }
 
 }  // namespace
@@ -446,35 +411,23 @@ void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, in
 
 uint32_t pair_blocks(uint32_t n_chunks) { return (n_chunks + PK_WAVES - 1) / PK_WAVES; }
 
-void launch_classify(xm_ctx* ctx, const xm_catalog* A, const PairParams& pp, Chunk* list,
-                     uint32_t* counts) {
-    if (A->n_chunks == 0) return;
-    const uint32_t nblk = uint32_t((A->n_chunks + 255) / 256);
-    launch_timed(ctx, "classify", [&] {
-        hipLaunchKernelGGL(k_classify, dim3(nblk), dim3(256), 0, ctx->stream,
-                           (const Chunk*)A->chunks, uint32_t(A->n_chunks), A->key, pp, list, counts);
+void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail) {
+    launch_timed(ctx, "region_scan", [&] {
+        hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1024), 0, ctx->stream,
+                           (const uint64_t*)ro.counters, ro.n_regions, ro.region_cap, prefix, tail);
     });
 }
 
 void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                      const Chunk* list, uint32_t ns, uint32_t nd, const RegionOut& ro) {
-    if (B->n == 0) return;
-    const Row64* arow = static_cast<const Row64*>(A->row);
-    const Row64* brow = static_cast<const Row64*>(B->row);
-    if (ns) {
-        launch_timed(ctx, "pair_sparse", [&] {
-            hipLaunchKernelGGL(k_pair_sparse, dim3(pair_blocks(ns)), dim3(PK_THREADS), 0,
-                               ctx->stream, list, ns, A->key, arow, B->key, brow, B->zoff, B->zmin,
-                               B->zmax, pp, ro);
-        });
-    }
-    if (nd) {
-        launch_timed(ctx, "pair_dense", [&] {
-            hipLaunchKernelGGL(k_pair_dense, dim3(pair_blocks(nd)), dim3(PK_THREADS), 0,
-                               ctx->stream, list + (A->n_chunks - nd), nd, A->key, arow, B->key,
-                               brow, B->zoff, B->zmin, B->zmax, pp, ro);
-        });
-    }
+                      const RegionOut& ro) {
+    if (B->n == 0 || A->max_chunks == 0) return;
+    launch_timed(ctx, "pair_fp64", [&] {
+        hipLaunchKernelGGL(k_pair_fp64, dim3(pair_blocks(uint32_t(A->max_chunks))), dim3(PK_THREADS),
+                           0, ctx->stream, (const Chunk*)A->chunks,
+                           (const uint32_t*)A->n_chunks_dev, A->key,
+                           static_cast<const Row64*>(A->row), B->key,
+                           static_cast<const Row64*>(B->row), B->zoff, B->zmin, B->zmax, pp, ro);
+    });
 }
 
 }  // namespace xm
