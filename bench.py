@@ -33,10 +33,10 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 # algorithmic HBM bytes per source for the memory-bound kernels of a catalog build
 BYTES_PER_SOURCE = {
-    "radix_scatter": 24.0,   # read key 8 + index 4, write key 8 + index 4
-    "radix_count": 8.0,      # read key
-    "make_keys": 36.0,       # read 24-B record, write key 8 + index 4
-    "gather": 60.0,          # read index 4 + 24-B record, write id 8 + xyz 24
+    "radix_scatter": 16.0,   # read packed word 8, write packed word 8
+    "radix_count": 8.0,      # read packed word
+    "make_keys": 32.0,       # read 24-B record, write packed word 8
+    "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
 }
 KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
            "scan_apply", "gather", "zone_offsets", "chunk_counts", "chunk_fill", "fill",
@@ -77,7 +77,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-dec-band", type=float, default=6.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 1 GiB (for scaling PMC byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": 1 << 30,
+                          "bytes_written": 1 << 30, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
