@@ -47,6 +47,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, profile="full_sky_50m_fp64.md"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (already scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    in_table = False
+    for line in open(path):
+        if line.startswith("## "):
+            in_table = "HBM bytes per dispatch" in line
+            continue
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if in_table and len(cells) == 5 and cells[0] == "k_" + kernel:
+            try:
+                return (float(cells[3]) + float(cells[4])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(A, B, r, dec_band, threads):
     """The CPU oracle (FP64 restatement of the reference zone join, dense zone-pair
     tests with the reference's box prefilter) on a bounded sample: both catalogs
@@ -158,7 +178,7 @@ def main():
         per_launch = BYTES_PER_SOURCE.get(dom, 0.0) * args.n
         ach = per_launch / (ktimes[dom]["avg_ms"] / 1e3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                 "algorithmic_bytes_per_launch": per_launch}
 
     sort_ms = sum(ktimes.get(k, {}).get("total_ms", 0.0)
