@@ -59,6 +59,8 @@ typedef struct {
     uint64_t n_retries;      /* result-buffer overflows that forced a re-run */
     uint64_t band_hits;      /* FP32 filter only: pairs inside the guard band (FP64 rechecked) */
     double join_ms;          /* device time of the join (HIP events), including re-runs */
+    double max_dist_err_arcsec; /* FP32 filter only: largest |chord_32 - chord_64| of a
+                                   match, in arcsec (the result holds the FP64 distance) */
 } xm_match_stats;
 
 /* ---- context: replaces CudaManager/CudaContext (CudaManager.cpp:8-65,

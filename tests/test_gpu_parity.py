@@ -285,3 +285,80 @@ def test_coarse_ra_keys_do_not_change_the_set(gpu_ctx, ra_bits):
     assert_same(got, ref)
     o = np.lexsort((got2[0], got2[1]))
     assert np.array_equal(got2[1][o], ref[0]) and np.array_equal(got2[0][o], ref[1])
+
+
+# ---------------------------------------------------------------- FP32 filter
+F32 = xm.FP32_FILTER_FP64_RECHECK
+
+
+def test_fp32_filter_small_patch_identical(gpu_ctx):
+    A, B, r = synth.config_small_patch(seed=1)
+    got, stats = gpu_join(gpu_ctx, A, B, r, precision=F32)
+    ref, _ = O.match(A, B, r)
+    assert_same(got, ref)  # distances are the FP64 ones
+    assert stats["band_hits"] >= 0 and stats["max_dist_err_arcsec"] < 0.1
+
+
+@pytest.mark.parametrize("where", ["seam", "north_pole", "south_pole", "dense_cluster", "mid"])
+def test_fp32_filter_edge_geometry(gpu_ctx, where):
+    rng = np.random.default_rng(abs(hash(where)) % 2**32 + 1)
+    n = 3000
+    if where == "seam":
+        ra = np.where(rng.random(n) < 0.5, rng.uniform(0, 0.01, n),
+                      rng.uniform(359.99, 360, n) % 360)
+        dec = rng.uniform(-0.02, 0.02, n)
+    elif where == "north_pole":
+        ra, dec = synth.uniform_patch(rng, n, 0, 360, 89.97, 90.0)
+    elif where == "south_pole":
+        ra, dec = synth.uniform_patch(rng, n, 0, 360, -90.0, -89.97)
+    elif where == "dense_cluster":
+        ra, dec = synth.uniform_patch(rng, n, 150.0, 150.003, 20.0, 20.0003)
+    else:
+        ra, dec = synth.uniform_patch(rng, n, 200, 200.04, 45, 45.03)
+    a = synth.make_objs(np.arange(n), ra, dec)
+    bra, bdec = synth.perturb(rng, ra, dec, 2.0)
+    b = synth.make_objs(10**9 + np.arange(n), bra, bdec)
+    for r in (1.5 / 3600, 4 / 3600, 60 / 3600):
+        got, _ = gpu_join(gpu_ctx, a, b, r, precision=F32)
+        assert_same(got, O.brute(a, b, r))
+
+
+def test_fp32_filter_boundary_pairs(gpu_ctx):
+    # pairs at r(1 + eps), eps down to 2^-50, sit inside the guard band: only the FP64
+    # recheck can decide them, and it must decide them as the oracle does
+    r = 1.0 / 3600
+    recs_a, recs_b = [], []
+    k = 0
+    for dec0 in (0.0, 0.3, 37.0, -62.0, 88.0):
+        for ra0 in (0.0, 45.0, 180.0, 359.9999999):
+            for e in (-2.0**-20, -2.0**-30, -2.0**-45, -2.0**-50, 0.0, 2.0**-50, 2.0**-45,
+                      2.0**-30, 2.0**-20):
+                dist = r * (1 + e)
+                c = np.cos(np.radians(dec0))
+                dra = np.degrees(2 * np.arcsin(np.sin(np.radians(dist) / 2) / c))
+                for (pa, pb) in (((ra0, dec0), (ra0, dec0 + dist)),
+                                 ((ra0, dec0), ((ra0 + dra) % 360, dec0)),
+                                 ((ra0, dec0), ((ra0 - dra) % 360, dec0))):
+                    recs_a.append((k, pa[0], pa[1]))
+                    recs_b.append((10**9 + k, pb[0], pb[1]))
+                    k += 1
+    a = np.array(recs_a, dtype=synth.OBJ_DTYPE)
+    b = np.array(recs_b, dtype=synth.OBJ_DTYPE)
+    b["dec"] = np.clip(b["dec"], -90, 90)
+    got, stats = gpu_join(gpu_ctx, a, b, r, precision=F32)
+    assert_same(got, O.match(a, b, r)[0])
+    assert stats["band_hits"] > 0
+
+
+def test_fp32_filter_full_sky_2m_same_as_fp64(gpu_ctx):
+    A, B, r = synth.config_full_sky(seed=7, n_a=2_000_000, n_b=2_000_000)
+    ca, cb = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
+    r64 = gpu_ctx.match(ca, cb, r)
+    r32 = gpu_ctx.match(ca, cb, r, precision=F32)
+    g64, g32 = r64.to_host(), r32.to_host()
+    s32 = r32.stats.as_dict()
+    r64.close(); r32.close(); ca.close(); cb.close()
+    assert np.array_equal(g64[0], g32[0]) and np.array_equal(g64[1], g32[1])
+    assert np.array_equal(g64[2], g32[2])  # bit-identical FP64 distances
+    assert_same(g64, O.match(A, B, r)[0])
+    assert s32["max_dist_err_arcsec"] < 0.1

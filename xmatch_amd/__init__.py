@@ -48,7 +48,7 @@ class XmatchError(RuntimeError):
 class MatchStats(ctypes.Structure):
     _fields_ = [("n_matches", ctypes.c_uint64), ("n_chunks", ctypes.c_uint64),
                 ("n_retries", ctypes.c_uint64), ("band_hits", ctypes.c_uint64),
-                ("join_ms", ctypes.c_double)]
+                ("join_ms", ctypes.c_double), ("max_dist_err_arcsec", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

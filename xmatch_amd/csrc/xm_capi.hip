@@ -88,6 +88,20 @@ PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, const xm_catalog*
     pp.atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
     pp.chord2 = chord2_of_radius(radius_deg);
     pp.gran = std::max(A->ra_gran, B->ra_gran);
+    // FP32 filter thresholds.  Unit-vector components rounded to FP32 move by at
+    // most 2^-25; the FP32 differences, products and sums add a relative 2^-24 each.
+    // So |sqrt(d2_32) - |u_a - u_b|| <= 2^-24 (sqrt(3) + 2.5 c) + O(2^-48) near the
+    // threshold chord c.  The band is twice that bound; (c + band)^2 is rounded up
+    // and (c - band)^2 down to FP32.
+    const double c = std::sqrt(pp.chord2);
+    const double band = 2.0 * std::ldexp(std::sqrt(3.0) + 2.5 * c + 1e-6, -24);
+    const double hi = (c + band) * (c + band);
+    const double lo = c > band ? (c - band) * (c - band) : 0.0;
+    float fhi = float(hi), flo = float(lo);
+    if (double(fhi) < hi) fhi = std::nextafter(fhi, INFINITY);
+    if (double(flo) > lo) flo = std::nextafter(flo, 0.0f);
+    pp.t_hi32 = fhi;
+    pp.t_lo32 = flo;
     pp.nz = int32_t(std::ceil(radius_deg / A->h_deg));  // Worker.cu:313
     pp.n_zones = A->n_zones;
     pp.dense_min = ctx->dense_min;
@@ -229,10 +243,11 @@ void catalog_free(xm_catalog* c) {
     xm_ctx* ctx = c->ctx;
     dfree(ctx, c->key);
     dfree(ctx, c->row);
+    dfree(ctx, c->row32);
     dfree(ctx, c->zoff);
     dfree(ctx, c->chunks);
     dfree(ctx, c->chunk_start);
-    c->key = nullptr; c->row = nullptr; c->zoff = nullptr; c->chunks = nullptr;
+    c->key = nullptr; c->row = nullptr; c->row32 = nullptr; c->zoff = nullptr; c->chunks = nullptr;
     c->chunk_start = nullptr; c->n_chunks_dev = nullptr;
 }
 
@@ -557,10 +572,20 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     std::memset(&st_local, 0, sizeof(st_local));
     xm_status st = guarded(ctx, [&] {
         check_join_args(a, b, radius_deg);
-        if (precision != XM_FP64_PLAIN)
-            throw InvalidArg{"precision mode not available in this build"};
+        if (precision != XM_FP64_PLAIN && precision != XM_FP32_FILTER_FP64_RECHECK)
+            throw InvalidArg{"unknown precision mode"};
+        const bool f32 = precision == XM_FP32_FILTER_FP64_RECHECK;
         xm_catalog* A = const_cast<xm_catalog*>(a);
+        xm_catalog* Bm = const_cast<xm_catalog*>(b);
         ensure_chunks(ctx, A);
+        if (f32) {
+            for (xm_catalog* c : {A, Bm}) {
+                if (c->row32 || c->n == 0) continue;
+                c->row32 = dalloc(ctx, c->n * sizeof(Row32));
+                launch_to_row32(ctx, static_cast<const Row64*>(c->row), c->n,
+                                static_cast<Row32*>(c->row32));
+            }
+        }
         const PairParams pp = make_params(ctx, A, b, radius_deg);
         uint64_t cap = capacity_hint ? capacity_hint : 2 * std::max(a->n, b->n);
         cap = std::max<uint64_t>(cap, 1024);
@@ -574,7 +599,7 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         ro.region_cap = std::max<uint64_t>(1, (cap + ro.n_regions - 1) / ro.n_regions);
         ro.counters = static_cast<uint64_t*>(
             dalloc(ctx, size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t)));
-        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(ro.n_regions) + 2) * sizeof(uint64_t)));
+        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(ro.n_regions) + 4) * sizeof(uint64_t)));
         uint64_t* prefix = scratch;
         uint64_t* tail = scratch + ro.n_regions;
         uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
@@ -589,14 +614,14 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             res->cap = slots;
             launch_fill_u64(ctx, ro.counters, size_t(ro.n_regions) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_pair_fp64(ctx, A, b, pp, ro);
+            launch_pair(ctx, A, b, pp, f32, ro);
             launch_region_scan(ctx, ro, prefix, tail);
             launch_compact(ctx, ro, prefix, res->id_a, res->id_b, res->dist);
             XM_HIP(hipEventRecord(ev1, ctx->stream));
-            XM_HIP(hipMemcpyAsync(pin, tail, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+            XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
             if (A->n_chunks_dev)
-                XM_HIP(hipMemcpyAsync(pin + 2, A->n_chunks_dev, sizeof(uint32_t),
+                XM_HIP(hipMemcpyAsync(pin + 4, A->n_chunks_dev, sizeof(uint32_t),
                                       hipMemcpyDeviceToHost, ctx->stream));
             dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
             ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr;
@@ -605,9 +630,13 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
             st_local.join_ms += ms;
             const uint64_t total = pin[0], most = pin[1];
-            st_local.n_chunks = A->n_chunks_dev ? reinterpret_cast<uint32_t*>(pin + 2)[0] : 0;
+            st_local.n_chunks = A->n_chunks_dev ? reinterpret_cast<uint32_t*>(pin + 4)[0] : 0;
             if (most <= ro.region_cap) {
                 res->n = total;
+                st_local.band_hits = pin[2];
+                double err = 0.0;
+                std::memcpy(&err, &pin[3], sizeof(double));
+                st_local.max_dist_err_arcsec = err * RAD2DEG * 3600.0;
                 break;
             }
             // overflow: every counter holds its region's exact total; re-run with

@@ -69,6 +69,7 @@ struct xm_catalog {
     int64_t ra_gran = 1;         // ra granularity of the keys, in 2^-24 turn quanta
     uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
     void* row = nullptr;         // xm::Row64[n]: FP64 unit vector + id, sorted order
+    void* row32 = nullptr;       // xm::Row32[n]: FP32 unit vectors, made on first FP32 join
     uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
     // A-side row chunks (built on first use as the A catalog of a join): the
     // table is sized for max_chunks, the true count stays on the device
@@ -97,6 +98,12 @@ constexpr int CHUNK_ROWS = 64;   // A rows per wavefront in the pair kernel
 struct Row64 {
     double x, y, z;
     int64_t id;
+};
+
+// FP32 unit vector of a row, for the FP32 filter (16 bytes, one aligned fetch).
+struct Row32 {
+    float x, y, z;
+    uint32_t pad;
 };
 
 // A run of at most CHUNK_ROWS consecutive A rows of one zone.  aq is the ra
@@ -163,6 +170,7 @@ struct PairParams {
     double atan_sin_theta;  // atan(sin(theta)), host-computed once
     double chord2;          // (2 sin(r/2))^2
     int64_t gran;           // coarser ra granularity of the two catalogs (quanta)
+    float t_lo32, t_hi32;   // FP32 filter: (c - band)^2 rounded down, (c + band)^2 up
     int32_t nz;             // ceil(r / h)
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
@@ -182,10 +190,13 @@ struct RegionOut {
     double* d2;      // chord^2 of each hit
 };
 uint32_t pair_blocks(uint32_t n_chunks);
-void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
-                      const PairParams& pp, const RegionOut& ro);
+// f32 = FP32 filter + FP64 recheck (needs A->row32 and B->row32), else FP64 plain.
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
+                 bool f32, const RegionOut& ro);
+void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
 // prefix[r] = hits of the regions before r (capped at region_cap each);
-// tail[0] = total hits, tail[1] = largest region count.
+// tail[0] = total hits, tail[1] = largest region count, tail[2] = FP32 band hits,
+// tail[3] = largest |chord_32 - chord_64| of a kept pair (double bits).
 void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail);
 // Packs the regions into contiguous arrays (region r's hits at prefix[r]) and
 // converts chord^2 to the FP64 distance in arcsec.

@@ -1,6 +1,6 @@
-// xm_pair.hip -- the zone-join pair kernels (FP64 plain form).
+// xm_pair.hip -- the zone-join pair kernel, FP64 plain and FP32 filter + FP64 recheck.
 //
-// Replace the per-zone-pair launch loop of Worker::Match and xmatch_kernel
+// Replaces the per-zone-pair launch loop of Worker::Match and xmatch_kernel
 // (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
 // into chunks of up to 64 consecutive rows of one zone; each wavefront owns one
 // chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
@@ -19,6 +19,11 @@
 //
 // Predicate (Worker.cu:88-97 with the seam bug D2 corrected), on FP64 unit vectors:
 //   d2 = fma(dz, dz, fma(dy, dy, dx*dx))  <  (2 sin(r/2))^2
+// FP32 filter form: the same chord on FP32 unit vectors keeps a pair when
+// d2_32 < (c + band)^2, c = sqrt(chord threshold), band >= twice the proven FP32
+// chord error; every kept pair is re-evaluated in FP64 exactly as above when it is
+// written, so the set and the distances are those of the FP64 form.
+//
 // Hits are compacted with ballot + mbcnt into a per-wave LDS stage; a full stage
 // (or the end of the chunk) costs one global atomic on the 64-bit append counter
 // of the block's output region.  A join's ~10^6 wavefronts spread over up to 1024
@@ -37,7 +42,7 @@ namespace {
 
 constexpr int PK_WAVES = 4;
 constexpr int PK_THREADS = PK_WAVES * WAVE;
-constexpr int STAGE = 128;
+constexpr int STAGE = 128;  // two wavefronts of hits
 constexpr int UNROLL = 4;
 constexpr uint32_t ONE_PASS_ROWS = 4 * WAVE;  // B zones up to this size are read in one pass
 
@@ -83,14 +88,26 @@ __device__ inline double dist_arcsec(double d2) {
     return 2.0 * asin(sqrt(d2) * 0.5) * RAD2DEG * 3600.0;
 }
 
-// The chord of the reference's exact test (Worker.cu:93-97) on FP64 unit vectors:
-// d2 = fma(dz, dz, fma(dy, dy, dx*dx)), identical on host and device.
-__device__ inline double chord2(double ax, double ay, double az, double bx, double by, double bz) {
-    const double dx = ax - bx;
-    const double dy = ay - by;
-    const double dz = az - bz;
+// The chord of the reference's exact test (Worker.cu:93-97) on unit vectors:
+// d2 = fma(dz, dz, fma(dy, dy, dx*dx)), in the precision T.  For T = double it is
+// the identical computation the host oracle performs.
+template <typename T>
+__device__ inline T chord2(T ax, T ay, T az, T bx, T by, T bz) {
+    const T dx = ax - bx;
+    const T dy = ay - by;
+    const T dz = az - bz;
     return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
+
+template <bool F32> struct Prec;
+template <> struct Prec<false> {
+    using T = double;
+    using Row = Row64;
+};
+template <> struct Prec<true> {
+    using T = float;
+    using Row = Row32;
+};
 
 struct Stage {
     uint32_t* ra;
@@ -102,40 +119,100 @@ struct Stage {
 struct Out {
     const Row64* __restrict__ arow;
     const Row64* __restrict__ brow;
-    unsigned long long* __restrict__ counter;  // this block's region counter
+    unsigned long long* __restrict__ counter;  // this block's region counter line
     uint64_t region_cap;
     int64_t* __restrict__ id_a;                // this block's region
     int64_t* __restrict__ id_b;
     double* __restrict__ d2;
+    double thr;                                // FP64 chord threshold
+    double t_lo;                               // FP32 form: below it the filter is sure
 };
 
-__device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro) {
+__device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro,
+                               const PairParams& pp) {
     const uint32_t r = blockIdx.x % ro.n_regions;
     const uint64_t base = uint64_t(r) * ro.region_cap;
     return Out{arow, brow, (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE),
-               ro.region_cap, ro.id_a + base, ro.id_b + base, ro.d2 + base};
+               ro.region_cap, ro.id_a + base, ro.id_b + base, ro.d2 + base, pp.chord2,
+               double(pp.t_lo32)};
 }
 
-// Write the wave's staged hits to its region: one atomic per call.
+// Write the wave's staged hits to its region: one atomic per call.  In the FP32
+// form each staged pair is re-evaluated in FP64 here and dropped if it fails; the
+// band-hit count and the largest |chord_32 - chord_64| go to the region's line.
+template <bool F32>
 __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
     wave_lds_sync();
+    bool keep[STAGE / WAVE];
+    double dd[STAGE / WAVE];
+    int64_t ia[STAGE / WAVE], ib[STAGE / WAVE];
+    uint64_t mk[STAGE / WAVE];
+    uint32_t total = 0, band = 0;
+    double err = 0.0;
+#pragma unroll
+    for (int k = 0; k < STAGE / WAVE; k++) {
+        const uint32_t e = lane + k * WAVE;
+        const bool valid = e < st.n;
+        keep[k] = valid;
+        dd[k] = 0.0;
+        ia[k] = 0;
+        ib[k] = 0;
+        if (valid) {
+            const Row64 a = o.arow[st.ra[e]];
+            const Row64 b = o.brow[st.rb[e]];
+            ia[k] = a.id;
+            ib[k] = b.id;
+            dd[k] = st.d2[e];
+            if (F32) {
+                const double d32 = dd[k];
+                dd[k] = chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z);
+                keep[k] = dd[k] < o.thr;
+                band += d32 >= o.t_lo ? 1u : 0u;
+                if (keep[k]) {
+                    const double e1 = __builtin_fabs(__builtin_sqrt(d32) - __builtin_sqrt(dd[k]));
+                    err = e1 > err ? e1 : err;
+                }
+            }
+        }
+        mk[k] = __ballot(keep[k]);
+        total += uint32_t(__popcll(mk[k]));
+    }
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(o.counter, (unsigned long long)st.n);
+    if (lane == 0 && total) base = atomicAdd(o.counter, (unsigned long long)total);
     base = __shfl(base, 0, WAVE);
-    for (uint32_t e = lane; e < st.n; e += WAVE) {
-        const uint64_t slot = uint64_t(base) + e;
-        if (slot < o.region_cap) {
-            o.id_a[slot] = o.arow[st.ra[e]].id;
-            o.id_b[slot] = o.brow[st.rb[e]].id;
-            o.d2[slot] = st.d2[e];
+    uint32_t before = 0;
+#pragma unroll
+    for (int k = 0; k < STAGE / WAVE; k++) {
+        if (keep[k]) {
+            const uint64_t slot = uint64_t(base) + before + mbcnt64(mk[k]);
+            if (slot < o.region_cap) {
+                o.id_a[slot] = ia[k];
+                o.id_b[slot] = ib[k];
+                o.d2[slot] = dd[k];
+            }
+        }
+        before += uint32_t(__popcll(mk[k]));
+    }
+    if (F32) {
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) {
+            band += __shfl_xor(band, d, WAVE);
+            const double e2 = __shfl_xor(err, d, WAVE);
+            err = e2 > err ? e2 : err;
+        }
+        if (lane == 0) {
+            if (band) atomicAdd(o.counter + 1, (unsigned long long)band);
+            if (err > 0.0)
+                atomicMax(o.counter + 2, (unsigned long long)__double_as_longlong(err));
         }
     }
     wave_lds_sync();
     st.n = 0;
 }
 
-// Append the lanes of hit mask m (A row ra, B row rb, per lane) to the stage;
+// Append the lanes of keep mask m (A row ra, B row rb, per lane) to the stage;
 // flush when it cannot take another full wave of hits.
+template <bool F32>
 __device__ inline void stage_hits(uint64_t m, double d2, uint32_t ra, uint32_t rb, uint32_t lane,
                                   Stage& st, const Out& o) {
     if ((m >> lane) & 1u) {
@@ -145,7 +222,7 @@ __device__ inline void stage_hits(uint64_t m, double d2, uint32_t ra, uint32_t r
         st.d2[pos] = d2;
     }
     st.n += uint32_t(__popcll(m));
-    if (st.n > uint32_t(STAGE - WAVE)) flush_stage(st, lane, o);
+    if (st.n > uint32_t(STAGE - WAVE)) flush_stage<F32>(st, lane, o);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of
@@ -178,20 +255,24 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ co
 }
 
 // --------------------------------------------------------------- sparse ----
+template <typename T>
 struct SparseCtx {
     uint32_t start, cnt;
     int32_t aq;
     const int32_t* s_aq;
-    const double *s_ax, *s_ay, *s_az;
-    double thr;
+    const T *s_ax, *s_ay, *s_az;
+    T thr;  // keep a pair when d2 < thr
 };
 
 // One B row per lane (row j, key); tested against the chunk rows within +-aq of
 // its ra if it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).
-__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key, int ni,
-                                  const int32_t* lq, const int32_t* hq, const int32_t* shift,
-                                  const Row64* __restrict__ brow, uint32_t lane, Stage& st,
-                                  const Out& o) {
+template <bool F32>
+__device__ inline void sparse_row(const SparseCtx<typename Prec<F32>::T>& x, uint32_t j, bool valid,
+                                  uint64_t key, int ni, const int32_t* lq, const int32_t* hq,
+                                  const int32_t* shift,
+                                  const typename Prec<F32>::Row* __restrict__ brow, uint32_t lane,
+                                  Stage& st, const Out& o) {
+    using T = typename Prec<F32>::T;
     const int32_t q = int32_t(key & RA_MASK);
     bool in = false;
     int32_t qb = 0;
@@ -211,74 +292,86 @@ __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, ui
         if (x.s_aq[c + step - 1] < lo_q) c += step;
     if (x.s_aq[c] < lo_q) c++;
     bool have_b = false;
-    double bx = 0.0, by = 0.0, bz = 0.0;
+    T bx = 0, by = 0, bz = 0;
     for (uint32_t k = 0;; k++) {
         const uint32_t cc = c + k;
         const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(WAVE) ? cc : 0] <= hi_q;
         if (!__ballot(act)) break;
-        double d2 = 0.0;
+        T d2 = 0;
         bool hit = false;
         if (act) {
             if (!have_b) {
-                const Row64 b = brow[j];
+                const auto b = brow[j];
                 bx = b.x;
                 by = b.y;
                 bz = b.z;
                 have_b = true;
             }
-            d2 = chord2(x.s_ax[cc], x.s_ay[cc], x.s_az[cc], bx, by, bz);
+            d2 = chord2<T>(x.s_ax[cc], x.s_ay[cc], x.s_az[cc], bx, by, bz);
             hit = d2 < x.thr;
         }
         const uint64_t m = __ballot(hit);
-        if (m) stage_hits(m, d2, x.start + cc, j, lane, st, o);
+        if (m) stage_hits<F32>(m, double(d2), x.start + cc, j, lane, st, o);
     }
 }
 
-// Dense mode: B rows [j0, j1) against the lane's A row (registers), the rows
-// arriving as wide scalar loads from a uniform base, one branch per UNROLL masks.
-__device__ inline void dense_range(uint32_t j0, uint32_t j1, double ax, double ay, double az,
-                                   uint32_t row, double thr, const Row64* __restrict__ brow,
+// ---------------------------------------------------------------- dense ----
+// B rows [j0, j1) against the lane's A row (registers), the rows arriving as wide
+// scalar loads from a uniform base, one branch per UNROLL masks.
+template <bool F32>
+__device__ inline void dense_range(uint32_t j0, uint32_t j1, typename Prec<F32>::T ax,
+                                   typename Prec<F32>::T ay, typename Prec<F32>::T az,
+                                   uint32_t row, typename Prec<F32>::T thr,
+                                   const typename Prec<F32>::Row* __restrict__ brow,
                                    uint32_t lane, Stage& st, const Out& o) {
+    using T = typename Prec<F32>::T;
+    using Row = typename Prec<F32>::Row;
     uint32_t j = j0;
     for (; j + UNROLL <= j1; j += UNROLL) {
-        const Row64* __restrict__ pb = brow + size_t(j);
-        double d2[UNROLL];
+        const Row* __restrict__ pb = brow + size_t(j);
+        T d2[UNROLL];
         uint64_t m[UNROLL];
         uint64_t any = 0;
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
-            d2[u] = chord2(ax, ay, az, pb[u].x, pb[u].y, pb[u].z);
+            d2[u] = chord2<T>(ax, ay, az, pb[u].x, pb[u].y, pb[u].z);
             m[u] = __ballot(d2[u] < thr);
             any |= m[u];
         }
         if (any) {
 #pragma unroll
             for (int u = 0; u < UNROLL; u++)
-                if (m[u]) stage_hits(m[u], d2[u], row, j + u, lane, st, o);
+                if (m[u]) stage_hits<F32>(m[u], double(d2[u]), row, j + u, lane, st, o);
         }
     }
     for (; j < j1; j++) {
-        const double d2 = chord2(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
+        const T d2 = chord2<T>(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
         const uint64_t m = __ballot(d2 < thr);
-        if (m) stage_hits(m, d2, row, j, lane, st, o);
+        if (m) stage_hits<F32>(m, double(d2), row, j, lane, st, o);
     }
 }
 
+// ----------------------------------------------------------------- join ----
 // One wavefront per A chunk; the chunk count lives on the device (*n_chunks), the
 // grid covers an upper bound and the waves past the count return at once.
-__global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
+// F32 selects the FP32 filter: tests read the catalogs' FP32 rows (arow_t/brow_t)
+// and keep pairs below t_hi32; the FP64 rows serve ids and the recheck.
+template <bool F32>
+__global__ __launch_bounds__(PK_THREADS) void k_pair(
     const Chunk* __restrict__ chunks, const uint32_t* __restrict__ n_chunks,
     const uint64_t* __restrict__ akey, const Row64* __restrict__ arow,
-    const uint64_t* __restrict__ bkey, const Row64* __restrict__ brow,
+    const typename Prec<F32>::Row* __restrict__ arow_t, const uint64_t* __restrict__ bkey,
+    const Row64* __restrict__ brow, const typename Prec<F32>::Row* __restrict__ brow_t,
     const uint32_t* __restrict__ bzoff, int32_t bzmin, int32_t bzmax, PairParams pp,
     RegionOut ro) {
+    using T = typename Prec<F32>::T;
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
     __shared__ int32_t s_aq[PK_WAVES][WAVE];
-    __shared__ double s_ax[PK_WAVES][WAVE];
-    __shared__ double s_ay[PK_WAVES][WAVE];
-    __shared__ double s_az[PK_WAVES][WAVE];
+    __shared__ T s_ax[PK_WAVES][WAVE];
+    __shared__ T s_ay[PK_WAVES][WAVE];
+    __shared__ T s_az[PK_WAVES][WAVE];
 
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
@@ -291,9 +384,9 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
 
     const uint32_t row = start + lane;
     int32_t qa = INT_MAX;
-    double ax = 10.0, ay = 10.0, az = 10.0;  // inactive lanes sit far off the sphere
+    T ax = T(10), ay = T(10), az = T(10);  // inactive lanes sit far off the sphere
     if (lane < cnt) {
-        const Row64 a = arow[row];
+        const auto a = arow_t[row];
         ax = a.x;
         ay = a.y;
         az = a.z;
@@ -303,6 +396,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
     const uint64_t q_hi = uint32_t(__builtin_amdgcn_readlane(qa, int(cnt - 1)));
     const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, za, pp.atan_sin_theta, pp.gran);
     const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
+    const T thr = F32 ? T(pp.t_hi32) : T(pp.chord2);
     if (!dense) {
         s_aq[w][lane] = qa;
         s_ax[w][lane] = ax;
@@ -311,9 +405,9 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
         wave_lds_sync();
     }
 
-    const SparseCtx x{start, cnt, int32_t(aq), s_aq[w], s_ax[w], s_ay[w], s_az[w], pp.chord2};
+    const SparseCtx<T> x{start, cnt, int32_t(aq), s_aq[w], s_ax[w], s_ay[w], s_az[w], thr};
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o = make_out(arow, brow, ro);
+    const Out o = make_out(arow, brow, ro, pp);
 
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
@@ -328,7 +422,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
             for (int t = 0; t < ni; t++) {
                 const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
                 const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
-                dense_range(j0, j1, ax, ay, az, row, pp.chord2, brow, lane, st, o);
+                dense_range<F32>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
             }
             continue;
         }
@@ -348,7 +442,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 if (b0 + k * WAVE >= b1) break;
                 const uint32_t j = b0 + k * WAVE + lane;
-                sparse_row(x, j, j < b1, kk[k], ni, lq, hq, shift, brow, lane, st, o);
+                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, brow_t, lane, st, o);
             }
         } else {
             for (int t = 0; t < ni; t++) {
@@ -358,43 +452,70 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair_fp64(
                     const uint32_t j = jb + lane;
                     const bool valid = j < j1;
                     const uint64_t key = valid ? bkey[j] : 0;
-                    sparse_row(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], brow, lane, st, o);
+                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], brow_t, lane,
+                                    st, o);
                 }
             }
         }
     }
-    if (st.n) flush_stage(st, lane, o);
+    if (st.n) flush_stage<F32>(st, lane, o);
 }
 
 // prefix[r] = sum of the capped counts of regions < r; tail[0] = total hits,
-// tail[1] = largest region count (uncapped).  One block.
+// tail[1] = largest region count (uncapped), tail[2] = band hits, tail[3] = the
+// largest chord error (double bits).  One block.
 __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict__ counters,
                                                       uint32_t n_regions, uint64_t region_cap,
                                                       uint64_t* __restrict__ prefix,
                                                       uint64_t* __restrict__ tail) {
     __shared__ uint64_t s_sum[1024];
     __shared__ uint64_t s_max[1024];
+    __shared__ uint64_t s_band[1024];
+    __shared__ uint64_t s_err[1024];
     const uint32_t t = threadIdx.x;
-    const uint64_t c = t < n_regions ? counters[size_t(t) * REGION_STRIDE] : 0;
+    const bool v = t < n_regions;
+    const uint64_t c = v ? counters[size_t(t) * REGION_STRIDE] : 0;
     s_sum[t] = c < region_cap ? c : region_cap;
     s_max[t] = c;
+    s_band[t] = v ? counters[size_t(t) * REGION_STRIDE + 1] : 0;
+    s_err[t] = v ? counters[size_t(t) * REGION_STRIDE + 2] : 0;
     __syncthreads();
-    // Hillis-Steele inclusive scan of the capped counts; max by tree reduction
+    // Hillis-Steele inclusive scan of the capped counts; the rest by tree reduction
     for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint64_t v = t >= d ? s_sum[t - d] : 0;
+        const uint64_t x = t >= d ? s_sum[t - d] : 0;
         __syncthreads();
-        s_sum[t] += v;
+        s_sum[t] += x;
         __syncthreads();
     }
     for (uint32_t d = 512; d >= 1; d >>= 1) {
-        if (t < d && s_max[t + d] > s_max[t]) s_max[t] = s_max[t + d];
+        if (t < d) {
+            if (s_max[t + d] > s_max[t]) s_max[t] = s_max[t + d];
+            if (s_err[t + d] > s_err[t]) s_err[t] = s_err[t + d];
+            s_band[t] += s_band[t + d];
+        }
         __syncthreads();
     }
-    if (t < n_regions) prefix[t] = s_sum[t] - (c < region_cap ? c : region_cap);
+    if (v) prefix[t] = s_sum[t] - (c < region_cap ? c : region_cap);
     if (t == 0) {
         tail[0] = s_sum[1023];
         tail[1] = s_max[0];
+        tail[2] = s_band[0];
+        tail[3] = s_err[0];
     }
+}
+
+// FP64 row -> FP32 row (round to nearest), for the FP32 filter.
+__global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, uint64_t n,
+                                                  Row32* __restrict__ out) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Row64 r = in[i];
+    Row32 o;
+    o.x = float(r.x);
+    o.y = float(r.y);
+    o.z = float(r.z);
+    o.pad = 0u;
+    out[i] = o;
 }
 
 }  // namespace
@@ -418,16 +539,35 @@ void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint
     });
 }
 
-void launch_pair_fp64(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                      const RegionOut& ro) {
-    if (B->n == 0 || A->max_chunks == 0) return;
-    launch_timed(ctx, "pair_fp64", [&] {
-        hipLaunchKernelGGL(k_pair_fp64, dim3(pair_blocks(uint32_t(A->max_chunks))), dim3(PK_THREADS),
-                           0, ctx->stream, (const Chunk*)A->chunks,
-                           (const uint32_t*)A->n_chunks_dev, A->key,
-                           static_cast<const Row64*>(A->row), B->key,
-                           static_cast<const Row64*>(B->row), B->zoff, B->zmin, B->zmax, pp, ro);
+void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out) {
+    if (n == 0) return;
+    launch_timed(ctx, "to_row32", [&] {
+        hipLaunchKernelGGL(k_to_row32, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           in, n, out);
     });
+}
+
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
+                 bool f32, const RegionOut& ro) {
+    if (B->n == 0 || A->max_chunks == 0) return;
+    const dim3 grid(pair_blocks(uint32_t(A->max_chunks)));
+    const Row64* arow = static_cast<const Row64*>(A->row);
+    const Row64* brow = static_cast<const Row64*>(B->row);
+    if (f32) {
+        launch_timed(ctx, "pair_fp32", [&] {
+            hipLaunchKernelGGL(k_pair<true>, grid, dim3(PK_THREADS), 0, ctx->stream,
+                               (const Chunk*)A->chunks, (const uint32_t*)A->n_chunks_dev, A->key,
+                               arow, static_cast<const Row32*>(A->row32), B->key, brow,
+                               static_cast<const Row32*>(B->row32), B->zoff, B->zmin, B->zmax, pp,
+                               ro);
+        });
+    } else {
+        launch_timed(ctx, "pair_fp64", [&] {
+            hipLaunchKernelGGL(k_pair<false>, grid, dim3(PK_THREADS), 0, ctx->stream,
+                               (const Chunk*)A->chunks, (const uint32_t*)A->n_chunks_dev, A->key,
+                               arow, arow, B->key, brow, brow, B->zoff, B->zmin, B->zmax, pp, ro);
+        });
+    }
 }
 
 }  // namespace xm
