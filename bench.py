@@ -87,8 +87,77 @@ def cpu_baseline(A, B, r, dec_band, threads):
     }
 
 
+def pair_digest(ida, idb, dist):
+    """Order-independent digest of a match set: sums of 64-bit mixes of the pairs
+    and of the distance bits."""
+    h = (ida.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ \
+        (idb.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F))
+    h ^= h >> np.uint64(29)
+    return {"n": int(len(ida)), "pairs": int(np.sum(h, dtype=np.uint64)),
+            "dist_bits": int(np.sum(dist.view(np.uint64), dtype=np.uint64))}
+
+
+def run_plane(args):
+    """Precision form: 200M x 200M with a galactic-plane-like density, r = h = 1",
+    FP64 plain against FP32 filter + FP64 recheck on the same catalogs."""
+    n = args.n if args.n != 50_000_000 else 200_000_000
+    r = 1.0 / 3600.0
+    ctx = xm.Context(0)
+    plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
+    t0 = time.time()
+    dA = ctx.synth_fresh(plane, n, args.seed * 1000 + 3, stream=0, id0=0)
+    dB = ctx.synth_copies(dA, 0.5, 0.3, plane, n, args.seed * 1000 + 3, stream=1)
+    log("generated %d x %d plane-density sources on the device in %.2f s" % (n, n, time.time() - t0))
+    ca = ctx.catalog(dA, r)
+    cb = ctx.catalog(dB, r)
+    out = {"metric": "pair-tests/s", "unit": "pair-tests/s", "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": "galactic-plane density zone join, %d x %d, r = h = 1 arcsec"
+                      % (n, n), "n_a": n, "n_b": n, "radius_arcsec": 1.0,
+                      "density": "1 + 20 exp(-b^2 / (2 (10 deg)^2)), pole (0, 30)",
+                      "b": "50% copies (0.3 arcsec) + 50% fresh"}}
+    w = ctx.work_counts(ca, cb, r)
+    out.update(w)
+    modes = {}
+    digests = {}
+    for name, prec in (("fp64_plain", xm.FP64_PLAIN), ("fp32_filter_fp64_recheck",
+                                                       xm.FP32_FILTER_FP64_RECHECK)):
+        for _ in range(args.warmup):
+            ctx.match(ca, cb, r, precision=prec).close()
+        ctx.synchronize()
+        ms = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            res = ctx.match(ca, cb, r, precision=prec)
+            ms.append(1e3 * (time.perf_counter() - t))
+            st = res.stats.as_dict()
+            last = res
+            if _ < args.steps - 1:
+                res.close()
+        ida, idb, dist = last.to_host(sort=False)
+        last.close()
+        digests[name] = pair_digest(ida, idb, dist)
+        del ida, idb, dist
+        join_ms = float(np.median(ms))
+        modes[name] = {"join_ms": join_ms, "device_join_ms": st["join_ms"],
+                       "pair_tests_per_s": w["w_cand"] / (join_ms / 1e3),
+                       "matches": st["n_matches"], "band_hits": st["band_hits"],
+                       "max_dist_err_arcsec": st["max_dist_err_arcsec"]}
+        log("%s: %.2f ms, %d matches" % (name, join_ms, st["n_matches"]))
+    out["modes"] = modes
+    out["identical_match_set"] = digests["fp64_plain"] == digests["fp32_filter_fp64_recheck"]
+    out["digest"] = digests["fp64_plain"]
+    out["fp32_over_fp64"] = (modes["fp64_plain"]["join_ms"] /
+                             modes["fp32_filter_fp64_recheck"]["join_ms"])
+    ca.close(); cb.close(); dA.free(); dB.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="full_sky", choices=["full_sky", "plane"],
+                    help="full_sky: the 50M x 50M FP64 join (default); plane: the 200M x 200M "
+                         "precision-form comparison")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -114,6 +183,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.config == "plane":
+        run_plane(args)
+        return
 
     t0 = time.time()
     A, B, r = synth.config_full_sky(seed=args.seed, n_a=args.n, n_b=args.n)

@@ -17,7 +17,7 @@ import weakref
 
 import numpy as np
 
-from .synth import OBJ_DTYPE, ARCSEC  # noqa: F401
+from .synth import OBJ_DTYPE, ARCSEC, B_ID_OFFSET  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxmatch_amd.so")
@@ -36,6 +36,7 @@ EXPORTED = [
     "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
     "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
     "xm_result_copy", "xm_result_destroy", "xm_work_counts",
+    "xm_synth_fresh", "xm_synth_copies",
 ]
 
 
@@ -52,6 +53,22 @@ class MatchStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Shape(ctypes.Structure):
+    """Synthetic-catalog shape (include/xmatch_synth.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("p", ctypes.c_double * 4)]
+
+
+SHAPE_SPHERE, SHAPE_PATCH, SHAPE_PLANE, SHAPE_CAP = range(4)
+
+
+def shape(kind, *p):
+    s = Shape()
+    s.kind = kind
+    for i, v in enumerate(p):
+        s.p[i] = v
+    return s
 
 
 _lib = None
@@ -99,6 +116,10 @@ def lib():
             "xm_result_destroy": ([vp], None),
             "xm_work_counts": ([vp, vp, vp, dbl, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                 ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_synth_fresh": ([vp, ctypes.POINTER(Shape), u64, u64, u64, ctypes.c_int64, vp],
+                               ctypes.c_int),
+            "xm_synth_copies": ([vp, vp, u64, dbl, dbl, ctypes.POINTER(Shape), u64, u64, u64,
+                                 ctypes.c_int64, vp], ctypes.c_int),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -180,6 +201,36 @@ class Context:
         self._check(lib().xm_device_alloc(self._h, max(arr.nbytes, 1), ctypes.byref(d)))
         self._check(lib().xm_memcpy_h2d(self._h, d, _ptr(arr), arr.nbytes))
         return DeviceBuffer(self, d, arr.nbytes, arr.dtype, arr.shape)
+
+    def alloc_records(self, n):
+        """Uninitialised device buffer of n 24-byte records."""
+        d = ctypes.c_void_p()
+        nbytes = int(n) * OBJ_DTYPE.itemsize
+        self._check(lib().xm_device_alloc(self._h, max(nbytes, 1), ctypes.byref(d)))
+        return DeviceBuffer(self, d, nbytes, OBJ_DTYPE, (int(n),))
+
+    def synth_fresh(self, shp, n, seed, stream=0, id0=0):
+        """n device records of shape shp (Philox-4x32-10 draws keyed by seed)."""
+        buf = self.alloc_records(n)
+        self._check(lib().xm_synth_fresh(self._h, ctypes.byref(shp), int(n), seed, stream,
+                                         int(id0), buf.ptr))
+        return buf
+
+    def synth_copies(self, a, frac_copy, sigma_arcsec, fresh, n, seed, stream=1,
+                     id0=B_ID_OFFSET):
+        """n device records: perturbed copies of records of DeviceBuffer a, then fresh."""
+        buf = self.alloc_records(n)
+        na = a.nbytes // OBJ_DTYPE.itemsize
+        self._check(lib().xm_synth_copies(self._h, a.ptr, na, frac_copy, sigma_arcsec,
+                                          ctypes.byref(fresh), int(n), seed, stream, int(id0),
+                                          buf.ptr))
+        return buf
+
+    def download(self, buf):
+        """Copy a DeviceBuffer back to a host numpy array."""
+        out = np.empty(buf.shape, dtype=buf.dtype)
+        self._check(lib().xm_memcpy_d2h(self._h, _ptr(out), buf.ptr, buf.nbytes))
+        return out
 
     def catalog(self, records, zone_height_deg):
         """Zone-sorted device catalog from host records (dtype OBJ_DTYPE) or a DeviceBuffer."""
