@@ -52,6 +52,24 @@ typedef struct xm_ctx xm_ctx;
 typedef struct xm_catalog xm_catalog;
 typedef struct xm_result xm_result;
 
+/* One segment of records for a many-segment catalog: the reference's Segment
+ * (BoostXmatch/BoostXmatch/Segment.h:22-41) before Sorter::Sort, i.e. a run of
+ * input records and the zone height it is indexed with. */
+typedef struct {
+    const xm_obj *records;   /* host or device pointer, per the creating call */
+    uint64_t n;
+    double zone_height_deg;
+} xm_segment;
+
+/* One join of a batch: segment seg_a of the A catalog against segment seg_b of
+ * the B catalog within radius_deg.  Replaces xmatch::Job (Job.h:16-29), which
+ * paired two Segment pointers with the search radius. */
+typedef struct {
+    uint32_t seg_a;
+    uint32_t seg_b;
+    double radius_deg;
+} xm_job;
+
 /* Work and timing figures of one xm_match call. */
 typedef struct {
     uint64_t n_matches;      /* pairs in the result */
@@ -108,10 +126,26 @@ xm_status xm_catalog_create(xm_ctx *ctx, const xm_obj *records, uint64_t n,
                             double zone_height_deg, xm_catalog **out);
 xm_status xm_catalog_create_device(xm_ctx *ctx, const xm_obj *d_records, uint64_t n,
                                    double zone_height_deg, xm_catalog **out);
+/* Many segments in one catalog, prepared together by one sort (the reference ran
+ * one Sorter::Sort per segment, BoostXmatch.cpp:250-263,284-297).  Each segment
+ * keeps its own zone height, zone table and rows; rows of segment s follow those
+ * of segment s-1.  At most 65536 segments and 2^32-2 records in all.  The
+ * _device variant takes device record pointers; the other copies host records. */
+xm_status xm_catalog_create_segments(xm_ctx *ctx, const xm_segment *segs, uint64_t n_segs,
+                                     xm_catalog **out);
+xm_status xm_catalog_create_segments_device(xm_ctx *ctx, const xm_segment *segs,
+                                            uint64_t n_segs, xm_catalog **out);
 void xm_catalog_destroy(xm_catalog *cat);
-/* n, zone height, occupied zone range [zmin, zmax] (zmin > zmax when empty). */
+/* n (all segments), zone height (of segment 0), occupied zone range [zmin, zmax]
+ * over all segments (zmin > zmax when empty). */
 xm_status xm_catalog_info(const xm_catalog *cat, uint64_t *n, double *zone_height_deg,
                           int32_t *zmin, int32_t *zmax);
+xm_status xm_catalog_segment_count(const xm_catalog *cat, uint64_t *n_segs);
+/* Segment seg: its first row in the catalog's sorted order, rows, zone height and
+ * occupied zone range.  Any output pointer may be NULL. */
+xm_status xm_catalog_segment_info(const xm_catalog *cat, uint64_t seg, uint64_t *first_row,
+                                  uint64_t *n, double *zone_height_deg, int32_t *zmin,
+                                  int32_t *zmax);
 /* Sorted-order export to host arrays of length n (any pointer may be NULL):
  * ids, FP64 unit vectors, zone ids.  For tests and tooling. */
 xm_status xm_catalog_export(xm_ctx *ctx, const xm_catalog *cat, int64_t *id, double *x,
@@ -125,6 +159,16 @@ xm_status xm_catalog_export(xm_ctx *ctx, const xm_catalog *cat, int64_t *id, dou
 xm_status xm_match(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b, double radius_deg,
                    xm_precision precision, uint64_t capacity_hint, xm_result **out,
                    xm_match_stats *stats);
+/* A batch of joins in one launch: job j joins segment jobs[j].seg_a of a with
+ * segment jobs[j].seg_b of b (both of the same zone height) within
+ * jobs[j].radius_deg.  Replaces JobManager's queue of Jobs, each run by
+ * Worker::Match (JobManager.cpp:5-41, Worker.cu:430-485).  The result holds the
+ * pairs of every job (unordered, like xm_match); xm_result_jobs gives the job of
+ * each pair and the count per job.  capacity_hint 0 = sum over jobs of
+ * 2 * max(nA, nB).  a may equal b. */
+xm_status xm_match_batch(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
+                         const xm_job *jobs, uint64_t n_jobs, xm_precision precision,
+                         uint64_t capacity_hint, xm_result **out, xm_match_stats *stats);
 xm_status xm_result_count(const xm_result *res, uint64_t *n);
 /* Device pointers to the n result triples (valid until xm_result_destroy). */
 xm_status xm_result_device_ptrs(const xm_result *res, const int64_t **id_a,
@@ -132,6 +176,10 @@ xm_status xm_result_device_ptrs(const xm_result *res, const int64_t **id_a,
 /* Copy the result to host arrays of length >= count (pointers may be NULL). */
 xm_status xm_result_copy(xm_ctx *ctx, const xm_result *res, int64_t *id_a, int64_t *id_b,
                          double *dist_arcsec);
+/* Batch results: job index of each pair (host array of length count) and pairs
+ * per job (host array of length n_jobs); either may be NULL. */
+xm_status xm_result_jobs(xm_ctx *ctx, const xm_result *res, uint32_t *job, uint64_t *job_counts);
+xm_status xm_result_job_device_ptr(const xm_result *res, const uint32_t **job);
 void xm_result_destroy(xm_result *res);
 
 /* ---- work counts, computed on the host from the catalogs' sorted keys and
@@ -142,6 +190,10 @@ void xm_result_destroy(xm_result *res);
 xm_status xm_work_counts(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
                          double radius_deg, uint64_t *w_ref, uint64_t *w_cand,
                          uint64_t *w_tile);
+/* The same for one job of a batch (its two segments). */
+xm_status xm_work_counts_job(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
+                             const xm_job *job, uint64_t *w_ref, uint64_t *w_cand,
+                             uint64_t *w_tile);
 
 /* ---- version */
 int xm_abi_version(void);

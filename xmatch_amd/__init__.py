@@ -9,7 +9,8 @@ Reference interface mirrored (paths relative to the reference repository):
   Segment + Sorter::Sort   BoostXmatch/BoostXmatch/Segment.h:22-41, Sorter.cu:141-185
                            -> Catalog
   Job + Worker::Match      Job.h:16-29, Worker.cu:243-428 -> Context.match
-  JobManager cross product JobManager.cpp:5-13           -> xmatch_amd.jobs
+  JobManager cross product JobManager.cpp:5-13           -> cross_product_jobs +
+                                                            Context.match_batch
 """
 import ctypes
 import os
@@ -36,6 +37,9 @@ EXPORTED = [
     "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
     "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
     "xm_result_copy", "xm_result_destroy", "xm_work_counts",
+    "xm_catalog_create_segments", "xm_catalog_create_segments_device",
+    "xm_catalog_segment_count", "xm_catalog_segment_info", "xm_match_batch",
+    "xm_result_jobs", "xm_result_job_device_ptr", "xm_work_counts_job",
     "xm_synth_fresh", "xm_synth_copies",
 ]
 
@@ -53,6 +57,33 @@ class MatchStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Segment(ctypes.Structure):
+    """xm_segment: records (host or device pointer), count, zone height."""
+    _fields_ = [("records", ctypes.c_void_p), ("n", ctypes.c_uint64),
+                ("zone_height_deg", ctypes.c_double)]
+
+
+class Job(ctypes.Structure):
+    """xm_job: segment of A x segment of B within radius_deg."""
+    _fields_ = [("seg_a", ctypes.c_uint32), ("seg_b", ctypes.c_uint32),
+                ("radius_deg", ctypes.c_double)]
+
+
+def cross_product_jobs(n_seg_a, n_seg_b, radius_deg):
+    """Every A segment against every B segment, A-major, as the reference's
+    JobManager builds its queue (JobManager.cpp:5-13)."""
+    return [(a, b, radius_deg) for a in range(n_seg_a) for b in range(n_seg_b)]
+
+
+def _job_array(jobs):
+    arr = (Job * max(len(jobs), 1))()
+    for i, (a, b, r) in enumerate(jobs):
+        arr[i].seg_a = int(a)
+        arr[i].seg_b = int(b)
+        arr[i].radius_deg = float(r)
+    return arr
 
 
 class Shape(ctypes.Structure):
@@ -116,6 +147,19 @@ def lib():
             "xm_result_destroy": ([vp], None),
             "xm_work_counts": ([vp, vp, vp, dbl, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                 ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_catalog_create_segments": ([vp, ctypes.POINTER(Segment), u64, pvp], ctypes.c_int),
+            "xm_catalog_create_segments_device": ([vp, ctypes.POINTER(Segment), u64, pvp],
+                                                  ctypes.c_int),
+            "xm_catalog_segment_count": ([vp, ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_catalog_segment_info": ([vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                         ctypes.POINTER(dbl), ctypes.POINTER(i32),
+                                         ctypes.POINTER(i32)], ctypes.c_int),
+            "xm_match_batch": ([vp, vp, vp, ctypes.POINTER(Job), u64, ctypes.c_int, u64, pvp,
+                                ctypes.POINTER(MatchStats)], ctypes.c_int),
+            "xm_result_jobs": ([vp, vp, vp, vp], ctypes.c_int),
+            "xm_result_job_device_ptr": ([vp, pvp], ctypes.c_int),
+            "xm_work_counts_job": ([vp, vp, vp, ctypes.POINTER(Job), ctypes.POINTER(u64),
+                                    ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
             "xm_synth_fresh": ([vp, ctypes.POINTER(Shape), u64, u64, u64, ctypes.c_int64, vp],
                                ctypes.c_int),
             "xm_synth_copies": ([vp, vp, u64, dbl, dbl, ctypes.POINTER(Shape), u64, u64, u64,
@@ -236,6 +280,12 @@ class Context:
         """Zone-sorted device catalog from host records (dtype OBJ_DTYPE) or a DeviceBuffer."""
         return Catalog(self, records, zone_height_deg)
 
+    def catalog_segments(self, segments, zone_height_deg):
+        """One catalog of many segments, prepared by one sort.  segments: host record
+        arrays (dtype OBJ_DTYPE) or DeviceBuffers (not mixed); zone_height_deg: one
+        value or one per segment."""
+        return Catalog(self, list(segments), zone_height_deg)
+
     def match(self, a, b, radius_deg, precision=FP64_PLAIN, capacity_hint=0):
         r = ctypes.c_void_p()
         stats = MatchStats()
@@ -243,10 +293,27 @@ class Context:
                                    capacity_hint, ctypes.byref(r), ctypes.byref(stats)))
         return Result(self, r, stats)
 
+    def match_batch(self, a, b, jobs, precision=FP64_PLAIN, capacity_hint=0):
+        """One launch over jobs [(seg_a, seg_b, radius_deg), ...]; the result carries
+        the job of every pair (Result.jobs)."""
+        arr = _job_array(jobs)
+        r = ctypes.c_void_p()
+        stats = MatchStats()
+        self._check(lib().xm_match_batch(self._h, a.handle, b.handle, arr, len(jobs), precision,
+                                         capacity_hint, ctypes.byref(r), ctypes.byref(stats)))
+        return Result(self, r, stats, n_jobs=len(jobs))
+
     def work_counts(self, a, b, radius_deg):
         w = [ctypes.c_uint64() for _ in range(3)]
         self._check(lib().xm_work_counts(self._h, a.handle, b.handle, radius_deg,
                                          *(ctypes.byref(x) for x in w)))
+        return {"w_ref": w[0].value, "w_cand": w[1].value, "w_tile": w[2].value}
+
+    def work_counts_job(self, a, b, job):
+        arr = _job_array([job])
+        w = [ctypes.c_uint64() for _ in range(3)]
+        self._check(lib().xm_work_counts_job(self._h, a.handle, b.handle, arr,
+                                             *(ctypes.byref(x) for x in w)))
         return {"w_ref": w[0].value, "w_cand": w[1].value, "w_tile": w[2].value}
 
 
@@ -264,7 +331,32 @@ class Catalog:
     def __init__(self, ctx, records, zone_height_deg):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
-        if isinstance(records, DeviceBuffer):
+        if isinstance(records, list):
+            self._keep = []
+            n = len(records)
+            hs = (list(zone_height_deg) if np.ndim(zone_height_deg) else
+                  [float(zone_height_deg)] * n)
+            segs = (Segment * max(n, 1))()
+            on_device = n > 0 and isinstance(records[0], DeviceBuffer)
+            for i, r in enumerate(records):
+                if isinstance(r, DeviceBuffer) != on_device:
+                    raise TypeError("segments must be all host arrays or all DeviceBuffers")
+                if on_device:
+                    segs[i].records = r.ptr
+                    segs[i].n = r.nbytes // OBJ_DTYPE.itemsize
+                else:
+                    rec = np.ascontiguousarray(r)
+                    if rec.dtype != OBJ_DTYPE:
+                        raise TypeError("records must have dtype %r" % (OBJ_DTYPE,))
+                    self._keep.append(rec)
+                    segs[i].records = rec.ctypes.data if rec.size else None
+                    segs[i].n = rec.size
+                segs[i].zone_height_deg = hs[i]
+            fn = (lib().xm_catalog_create_segments_device if on_device
+                  else lib().xm_catalog_create_segments)
+            st = fn(ctx.handle, segs, n, ctypes.byref(self._h))
+            self._keep = None
+        elif isinstance(records, DeviceBuffer):
             n = records.nbytes // OBJ_DTYPE.itemsize
             st = lib().xm_catalog_create_device(ctx.handle, records.ptr, n, zone_height_deg,
                                                 ctypes.byref(self._h))
@@ -290,6 +382,21 @@ class Catalog:
                                               ctypes.byref(z0), ctypes.byref(z1)))
         return {"n": n.value, "zone_height_deg": h.value, "zmin": z0.value, "zmax": z1.value}
 
+    @property
+    def segment_count(self):
+        n = ctypes.c_uint64()
+        self.ctx._check(lib().xm_catalog_segment_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def segment_info(self, s):
+        f, n, h = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        z0, z1 = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx._check(lib().xm_catalog_segment_info(self._h, s, ctypes.byref(f), ctypes.byref(n),
+                                                      ctypes.byref(h), ctypes.byref(z0),
+                                                      ctypes.byref(z1)))
+        return {"first_row": f.value, "n": n.value, "zone_height_deg": h.value,
+                "zmin": z0.value, "zmax": z1.value}
+
     def export(self):
         n = self.info()["n"]
         out = {"id": np.empty(n, np.int64), "x": np.empty(n), "y": np.empty(n),
@@ -312,11 +419,30 @@ class Catalog:
 
 
 class Result:
-    def __init__(self, ctx, h, stats):
+    def __init__(self, ctx, h, stats, n_jobs=None):
         self.ctx = ctx
         self._h = h
         self.stats = stats
+        self.n_jobs = n_jobs
         ctx._children.add(self)
+
+    def jobs(self):
+        """Batch results: (job index of each pair, in to_host(sort=False) order,
+        pairs per job)."""
+        job = np.empty(self.count, np.uint32)
+        counts = np.empty(self.n_jobs or 0, np.uint64)
+        self.ctx._check(lib().xm_result_jobs(self.ctx.handle, self._h, _ptr(job), _ptr(counts)))
+        return job, counts
+
+    def by_job(self):
+        """Batch results split per job: a list of (id_a, id_b, dist) sorted by (id_a, id_b)."""
+        ida, idb, dist = self.to_host(sort=False)
+        job, counts = self.jobs()
+        o = np.lexsort((idb, ida, job))
+        ida, idb, dist, job = ida[o], idb[o], dist[o], job[o]
+        edges = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+        return [(ida[edges[j]:edges[j + 1]], idb[edges[j]:edges[j + 1]],
+                 dist[edges[j]:edges[j + 1]]) for j in range(len(counts))]
 
     @property
     def count(self):

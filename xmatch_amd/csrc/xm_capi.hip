@@ -20,8 +20,11 @@ using namespace xm;
 namespace {
 
 thread_local std::string g_tls_error = "no error";
-// pinned host scratch: region counters and their prefix sums
-constexpr size_t PINNED_BYTES = size_t(MAX_REGIONS) * (REGION_STRIDE + 1) * sizeof(uint64_t);
+// pinned host scratch: small device->host reads at the front, then a stage for
+// the job table of small batches (one plain join included)
+constexpr size_t PIN_JOB_OFFSET = 4096;
+constexpr size_t PIN_JOB_BYTES = 64 * 1024;
+constexpr size_t PINNED_BYTES = PIN_JOB_OFFSET + PIN_JOB_BYTES;
 
 template <typename F>
 xm_status guarded(xm_ctx* ctx, F&& f) {
@@ -61,33 +64,36 @@ int bit_length(uint32_t v) {
     return b;
 }
 
-// Build the A-side chunk table: zones in [zmin, zmax] split into runs of at most
+// Build the A-side chunk table: every segment zone split into runs of at most
 // CHUNK_ROWS rows.  No synchronisation: the table is sized for the upper bound
-// n / CHUNK_ROWS + zones and the kernels read the true count from the device.
+// n / CHUNK_ROWS + zones and the kernels read the true counts from the device.
 void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     if (A->chunks || A->n == 0) return;
-    const int32_t nzone = A->zmax - A->zmin + 1;
-    uint32_t* counts = static_cast<uint32_t*>(dalloc(ctx, (size_t(nzone) + 1) * sizeof(uint32_t)));
-    launch_fill_u32(ctx, counts + nzone, 1, 0u);
-    launch_chunk_counts(ctx, A->zoff, nzone, counts);
-    scan_exclusive_u32(ctx, counts, uint64_t(nzone) + 1);
-    const uint64_t bound = A->n / CHUNK_ROWS + uint64_t(nzone);
+    const uint64_t entries = A->n_zoff - 1;  // zone entries; the last zoff entry is n
+    uint32_t* counts = static_cast<uint32_t*>(dalloc(ctx, A->n_zoff * sizeof(uint32_t)));
+    launch_fill_u32(ctx, counts + entries, 1, 0u);
+    launch_chunk_counts(ctx, A->zoff, entries, counts);
+    scan_exclusive_u32(ctx, counts, A->n_zoff);
+    const uint64_t bound = A->n / CHUNK_ROWS + entries;
     Chunk* chunks = static_cast<Chunk*>(dalloc(ctx, std::max<uint64_t>(bound, 1) * sizeof(Chunk)));
-    launch_chunk_fill(ctx, A->zoff, counts, A->zmin, nzone, chunks);
+    launch_chunk_fill(ctx, A->zoff, counts, A->key, entries, chunks);
     A->chunks = chunks;
     A->max_chunks = bound;
     A->chunk_start = counts;
-    A->n_chunks_dev = counts + nzone;
 }
 
-PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B,
-                       double radius_deg) {
+// Upper bound on the chunks of segment s (rows / CHUNK_ROWS + zones).
+uint64_t seg_max_chunks(const SegDesc& s) {
+    return s.n ? uint64_t(s.n) / CHUNK_ROWS + uint64_t(s.zmax - s.zmin + 1) : 0;
+}
+
+PairParams make_params(const xm_ctx* ctx, const SegDesc& A, const SegDesc& B, double radius_deg) {
     PairParams pp;
     pp.theta_deg = radius_deg;
-    pp.h_deg = A->h_deg;
+    pp.h_deg = A.h_deg;
     pp.atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
     pp.chord2 = chord2_of_radius(radius_deg);
-    pp.gran = std::max(A->ra_gran, B->ra_gran);
+    pp.gran = std::max(A.ra_gran, B.ra_gran);
     // FP32 filter thresholds.  Unit-vector components rounded to FP32 move by at
     // most 2^-25; the FP32 differences, products and sums add a relative 2^-24 each.
     // So |sqrt(d2_32) - |u_a - u_b|| <= 2^-24 (sqrt(3) + 2.5 c) + O(2^-48) near the
@@ -102,20 +108,17 @@ PairParams make_params(const xm_ctx* ctx, const xm_catalog* A, const xm_catalog*
     if (double(flo) > lo) flo = std::nextafter(flo, 0.0f);
     pp.t_hi32 = fhi;
     pp.t_lo32 = flo;
-    pp.nz = int32_t(std::ceil(radius_deg / A->h_deg));  // Worker.cu:313
-    pp.n_zones = A->n_zones;
+    pp.nz = int32_t(std::ceil(radius_deg / A.h_deg));  // Worker.cu:313
+    pp.n_zones = A.n_zones;
     pp.dense_min = ctx->dense_min;
     return pp;
 }
 
-void check_join_args(const xm_catalog* a, const xm_catalog* b, double radius_deg) {
-    if (!a || !b) throw InvalidArg{"null catalog"};
+void check_radius(double radius_deg, double h_a, double h_b) {
     if (!(radius_deg > 0.0) || !std::isfinite(radius_deg) || radius_deg >= 90.0)
         throw InvalidArg{"radius must be in (0, 90) degrees"};
-    if (a->h_deg != b->h_deg)
-        throw InvalidArg{"catalogs were prepared with different zone heights"};
-    if (std::ceil(radius_deg / a->h_deg) > 1e6)
-        throw InvalidArg{"radius / zone height is too large"};
+    if (h_a != h_b) throw InvalidArg{"segments were prepared with different zone heights"};
+    if (std::ceil(radius_deg / h_a) > 1e6) throw InvalidArg{"radius / zone height is too large"};
 }
 
 }  // namespace
@@ -247,65 +250,243 @@ void catalog_free(xm_catalog* c) {
     dfree(ctx, c->zoff);
     dfree(ctx, c->chunks);
     dfree(ctx, c->chunk_start);
+    dfree(ctx, c->seg_dev);
     c->key = nullptr; c->row = nullptr; c->row32 = nullptr; c->zoff = nullptr; c->chunks = nullptr;
-    c->chunk_start = nullptr; c->n_chunks_dev = nullptr;
+    c->chunk_start = nullptr; c->seg_dev = nullptr;
 }
 
-void catalog_build(xm_ctx* ctx, const xm_obj* d_recs, uint64_t n, double h_deg, xm_catalog* cat) {
-    cat->ctx = ctx;
-    cat->n = n;
-    cat->h_deg = h_deg;
-    cat->n_zones = n_zones_of(h_deg);
-    cat->zmin = 0;
-    cat->zmax = -1;
-    if (n > 0) {
-        PackLayout pl = pack_layout(n, cat->n_zones);
-        if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
-        cat->ra_gran = int64_t(1) << (RA_BITS - pl.ra_bits);
-        uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-        uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-        int32_t* small = static_cast<int32_t*>(dalloc(ctx, 4 * sizeof(int32_t)));
-        launch_fill_i32(ctx, small, 1, INT_MAX);
-        launch_fill_i32(ctx, small + 1, 1, -1);
-        launch_fill_i32(ctx, small + 2, 1, 0);
-        launch_make_keys(ctx, d_recs, n, h_deg, cat->n_zones, pl, words, small,
-                         reinterpret_cast<uint32_t*>(small + 2));
-        int32_t* pin = static_cast<int32_t*>(ctx->pinned);
-        XM_HIP(hipMemcpyAsync(pin, small, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        sync(ctx);
-        const int32_t zmin = pin[0], zmax = pin[1];
-        const uint32_t nbad = uint32_t(pin[2]);
-        dfree(ctx, small);
-        if (nbad) {
-            dfree(ctx, words);
-            dfree(ctx, words_alt);
-            throw InvalidArg{std::to_string(nbad) +
-                             " record(s) with NaN or out-of-range coordinates "
-                             "(need 0 <= ra < 360, -90 <= dec <= 90)"};
-        }
-        cat->zmin = zmin;
-        cat->zmax = zmax;
-        // Sort on the zone and ra bits (from the digit holding the lowest ra bit; the
-        // row-index bits below it only order ties).  Zone digits whose bits agree
-        // between zmin and zmax are constant over the segment and skipped.
-        const int zone_lo = pl.ra_bits + pl.idx_bits;
-        const int hi_bit = zone_lo + bit_length(uint32_t(zmax));
-        uint64_t skip = 0;
-        for (int d = 0; d < 8; d++) {
-            if (d * 8 < zone_lo) continue;
-            const int zs = d * 8 - zone_lo;
-            if ((uint64_t(uint32_t(zmin)) >> zs) == (uint64_t(uint32_t(zmax)) >> zs))
-                skip |= uint64_t(1) << d;
-        }
-        radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit, skip);
-        dfree(ctx, words_alt);
-        cat->row = dalloc(ctx, n * sizeof(Row64));
-        launch_gather(ctx, d_recs, words, n, pl, static_cast<Row64*>(cat->row));
-        cat->key = words;
+// Segment table checks and row bases; returns the total row count.
+uint64_t seg_table(const std::vector<const xm_obj*>& recs, const std::vector<uint64_t>& ns,
+                   const std::vector<double>& hs, xm_catalog* cat) {
+    const size_t S = ns.size();
+    if (S == 0) throw InvalidArg{"a catalog needs at least one segment"};
+    if (S > MAX_SEGMENTS) throw InvalidArg{"more than 65536 segments in one catalog"};
+    uint64_t total = 0;
+    cat->seg.assign(S, SegDesc{});
+    for (size_t s = 0; s < S; s++) {
+        const double h = hs[s];
+        if (!(h > 0.0) || !std::isfinite(h) || h > 180.0)
+            throw InvalidArg{"zone height must be in (0, 180] degrees"};
+        if (180.0 / h > double(1 << 24)) throw InvalidArg{"zone height too small (more than 2^24 zones)"};
+        if (ns[s] && !recs[s]) throw InvalidArg{"null records"};
+        SegDesc& d = cat->seg[s];
+        d.recs = recs[s];
+        d.row_base = uint32_t(total);
+        d.n = uint32_t(ns[s]);
+        d.zmin = 0;
+        d.zmax = -1;
+        d.zoff_base = 0;
+        d.n_zones = n_zones_of(h);
+        d.h_deg = h;
+        d.ra_gran = 1;
+        d.zone_base = 0;
+        d.zone_bits = d.ra_bits = d.idx_bits = 0;
+        total += ns[s];
+        if (total >= 0xffffffffull) throw InvalidArg{"a catalog holds fewer than 2^32-1 records"};
     }
-    const int64_t ntab = int64_t(cat->zmax) - cat->zmin + 2;
-    cat->zoff = static_cast<uint32_t*>(dalloc(ctx, size_t(ntab) * sizeof(uint32_t)));
-    launch_zone_offsets(ctx, cat->key, n, cat->zmin, cat->zmax, cat->zoff);
+    return total;
+}
+
+// Zone tables of every segment from the sorted catalog keys, and the device copy
+// of the segment table with the final zone ranges.
+void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
+    uint64_t entries = 0;
+    for (auto& d : cat->seg) {
+        d.zoff_base = uint32_t(entries);
+        if (d.n) entries += uint64_t(d.zmax - d.zmin + 1);
+        d.recs = nullptr;
+    }
+    cat->n_zoff = entries + 1;
+    if (!cat->seg_dev) cat->seg_dev = static_cast<SegDesc*>(dalloc(ctx, cat->seg.size() * sizeof(SegDesc)));
+    XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), cat->seg.size() * sizeof(SegDesc),
+                          hipMemcpyHostToDevice, ctx->stream));
+    cat->zoff = static_cast<uint32_t*>(dalloc(ctx, cat->n_zoff * sizeof(uint32_t)));
+    launch_zone_offsets(ctx, cat->key, cat->n, cat->seg_dev, cat->n_zoff, cat->zoff);
+    // cat->seg stays as uploaded; the creating call synchronises before returning
+}
+
+// One segment: a single pass over the records packs zone | ra | row words with the
+// absolute zone (the zone range comes back with the same pass).
+void build_one(xm_ctx* ctx, xm_catalog* cat) {
+    SegDesc& d = cat->seg[0];
+    const uint64_t n = d.n;
+    if (n == 0) return;
+    PackLayout pl = pack_layout(n, d.n_zones);
+    if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
+    d.ra_gran = int64_t(1) << (RA_BITS - pl.ra_bits);
+    d.idx_bits = pl.idx_bits;
+    d.ra_bits = pl.ra_bits;
+    d.zone_bits = pl.zone_bits;
+    d.zone_base = 0;
+    uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    int32_t* small = static_cast<int32_t*>(dalloc(ctx, 4 * sizeof(int32_t)));
+    launch_fill_i32(ctx, small, 1, INT_MAX);
+    launch_fill_i32(ctx, small + 1, 1, -1);
+    launch_fill_i32(ctx, small + 2, 1, 0);
+    launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
+                     reinterpret_cast<uint32_t*>(small + 2));
+    int32_t* pin = static_cast<int32_t*>(ctx->pinned);
+    XM_HIP(hipMemcpyAsync(pin, small, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    const int32_t zmin = pin[0], zmax = pin[1];
+    const uint32_t nbad = uint32_t(pin[2]);
+    dfree(ctx, small);
+    if (nbad) {
+        dfree(ctx, words);
+        dfree(ctx, words_alt);
+        throw InvalidArg{std::to_string(nbad) +
+                         " record(s) with NaN or out-of-range coordinates "
+                         "(need 0 <= ra < 360, -90 <= dec <= 90)"};
+    }
+    d.zmin = zmin;
+    d.zmax = zmax;
+    // Sort on the zone and ra bits (from the digit holding the lowest ra bit; the
+    // row-index bits below it only order ties).  Zone digits whose bits agree
+    // between zmin and zmax are constant over the segment and skipped.
+    const int zone_lo = pl.ra_bits + pl.idx_bits;
+    const int hi_bit = zone_lo + bit_length(uint32_t(zmax));
+    uint64_t skip = 0;
+    for (int dg = 0; dg < 8; dg++) {
+        if (dg * 8 < zone_lo) continue;
+        const int zs = dg * 8 - zone_lo;
+        if ((uint64_t(uint32_t(zmin)) >> zs) == (uint64_t(uint32_t(zmax)) >> zs))
+            skip |= uint64_t(1) << dg;
+    }
+    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit, skip);
+    dfree(ctx, words_alt);
+    cat->row = dalloc(ctx, n * sizeof(Row64));
+    launch_gather(ctx, d.recs, words, n, pl, static_cast<Row64*>(cat->row));
+    cat->key = words;
+}
+
+// Many segments: pass 1 finds every segment's zone range, the host picks each
+// segment's layout (segment-relative zone, so small segments keep more ra bits),
+// pass 2 packs segment | zone | ra | row words, one radix sort orders them all.
+void build_many(xm_ctx* ctx, xm_catalog* cat) {
+    const uint32_t S = uint32_t(cat->seg.size());
+    const uint64_t n = cat->n;
+    if (n == 0) return;
+    cat->seg_dev = static_cast<SegDesc*>(dalloc(ctx, S * sizeof(SegDesc)));
+    XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice,
+                          ctx->stream));
+    int32_t* range = static_cast<int32_t*>(dalloc(ctx, size_t(S) * 3 * sizeof(int32_t)));
+    std::vector<int32_t> init(size_t(S) * 3);
+    for (uint32_t s = 0; s < S; s++) {
+        init[3 * s] = INT_MAX;
+        init[3 * s + 1] = -1;
+        init[3 * s + 2] = 0;
+    }
+    XM_HIP(hipMemcpyAsync(range, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    launch_seg_range(ctx, cat->seg_dev, S, n, range);
+    XM_HIP(hipMemcpyAsync(init.data(), range, init.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    sync(ctx);
+    dfree(ctx, range);
+    uint64_t nmax = 0;
+    for (uint32_t s = 0; s < S; s++) {
+        if (init[3 * s + 2])
+            throw InvalidArg{"segment " + std::to_string(s) +
+                             " holds record(s) with NaN or out-of-range coordinates "
+                             "(need 0 <= ra < 360, -90 <= dec <= 90)"};
+        nmax = std::max<uint64_t>(nmax, cat->seg[s].n);
+    }
+    const int seg_bits = bit_length_u64(uint64_t(S - 1));
+    const int idx_bits = bit_length_u64(nmax - 1);
+    int content = 0;
+    for (uint32_t s = 0; s < S; s++) {
+        SegDesc& d = cat->seg[s];
+        d.idx_bits = idx_bits;
+        if (d.n == 0) {
+            d.zone_bits = 0;
+            d.ra_bits = 1;
+            continue;
+        }
+        d.zmin = init[3 * s];
+        d.zmax = init[3 * s + 1];
+        d.zone_base = d.zmin;
+        d.zone_bits = bit_length_u64(uint64_t(d.zmax - d.zmin));
+        const int room = 64 - seg_bits - idx_bits - d.zone_bits;
+        d.ra_bits = std::min(std::min(room, RA_BITS), ctx->key_ra_bits);
+        if (d.ra_bits < 1) throw InvalidArg{"segments too large to pack zone, ra and row in 64 bits"};
+        d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
+        content = std::max(content, idx_bits + d.ra_bits + d.zone_bits);
+    }
+    const int seg_shift = seg_bits ? content : 64;
+    XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice,
+                          ctx->stream));
+    uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    launch_seg_words(ctx, cat->seg_dev, S, n, seg_shift, words);
+    radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits, 0);
+    dfree(ctx, words_alt);
+    cat->row = dalloc(ctx, n * sizeof(Row64));
+    launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row));
+    cat->key = words;
+    sync(ctx);  // the host segment table is rewritten next; its uploads must be done
+}
+
+void catalog_build(xm_ctx* ctx, const std::vector<const xm_obj*>& recs,
+                   const std::vector<uint64_t>& ns, const std::vector<double>& hs,
+                   xm_catalog* cat) {
+    cat->ctx = ctx;
+    cat->n = seg_table(recs, ns, hs, cat);
+    if (cat->seg.size() == 1) build_one(ctx, cat);
+    else build_many(ctx, cat);
+    finish_tables(ctx, cat);
+}
+
+// A batch of jobs on the device: JobDesc table, per-job wave counts scanned into
+// job_prefix (n_jobs + 1 entries, the last = all waves), and the host-side bound
+// on the waves that sizes the grid.
+struct JobTable {
+    JobDesc* jobs = nullptr;
+    uint32_t* prefix = nullptr;
+    uint32_t n_jobs = 0;
+    uint64_t max_waves = 0;
+    std::vector<JobDesc> host;  // upload source when the pinned stage is too small
+};
+
+// The table is uploaded asynchronously; t.host (or the context's pinned job stage)
+// must stay untouched until the stream is synchronised.
+void make_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jobs,
+               uint64_t n_jobs, JobTable& t) {
+    if (n_jobs == 0) throw InvalidArg{"no jobs"};
+    if (n_jobs >= 0xffffffffull) throw InvalidArg{"too many jobs"};
+    const bool pinned = n_jobs * sizeof(JobDesc) <= PIN_JOB_BYTES;
+    if (!pinned) t.host.resize(n_jobs);
+    JobDesc* h = pinned ? reinterpret_cast<JobDesc*>(static_cast<char*>(ctx->pinned) + PIN_JOB_OFFSET)
+                        : t.host.data();
+    uint64_t bound = 0;
+    for (uint64_t j = 0; j < n_jobs; j++) {
+        const xm_job& jb = jobs[j];
+        if (jb.seg_a >= A->seg.size() || jb.seg_b >= B->seg.size())
+            throw InvalidArg{"job " + std::to_string(j) + " names a segment the catalog lacks"};
+        const SegDesc& sa = A->seg[jb.seg_a];
+        const SegDesc& sb = B->seg[jb.seg_b];
+        check_radius(jb.radius_deg, sa.h_deg, sb.h_deg);
+        JobDesc& d = h[j];
+        d.pp = make_params(ctx, sa, sb, jb.radius_deg);
+        d.b_key_prefix = uint64_t(jb.seg_b) << KEY_SEG_SHIFT;
+        d.a_zoff_lo = sa.zoff_base;
+        d.a_zoff_hi = sa.zoff_base + (sa.n ? uint32_t(sa.zmax - sa.zmin + 1) : 0u);
+        d.b_zoff_base = sb.zoff_base;
+        d.b_zmin = sb.n ? sb.zmin : 0;
+        d.b_zmax = sb.n ? sb.zmax : -1;
+        d.a_chunk_base = 0;
+        if (sb.n) bound += seg_max_chunks(sa);
+    }
+    if (bound >= 0xffffffffull) throw InvalidArg{"batch too large (more than 2^32 wavefronts)"};
+    t.n_jobs = uint32_t(n_jobs);
+    t.max_waves = bound;
+    ensure_chunks(ctx, A);
+    t.jobs = static_cast<JobDesc*>(dalloc(ctx, n_jobs * sizeof(JobDesc)));
+    t.prefix = static_cast<uint32_t*>(dalloc(ctx, (n_jobs + 1) * sizeof(uint32_t)));
+    XM_HIP(hipMemcpyAsync(t.jobs, h, n_jobs * sizeof(JobDesc), hipMemcpyHostToDevice, ctx->stream));
+    launch_fill_u32(ctx, t.prefix, n_jobs + 1, 0u);
+    if (A->chunk_start) launch_job_prep(ctx, t.jobs, t.n_jobs, A->chunk_start, t.prefix);
+    scan_exclusive_u32(ctx, t.prefix, n_jobs + 1);
 }
 
 }  // namespace
@@ -464,33 +645,49 @@ xm_status xm_memcpy_d2h(xm_ctx* ctx, void* dst, const void* src, size_t bytes) {
     });
 }
 
-static xm_status catalog_create_impl(xm_ctx* ctx, const xm_obj* recs, bool on_device, uint64_t n,
-                                     double zone_height_deg, xm_catalog** out) {
-    if (!ctx || !out) return XM_EINVAL;
+static xm_status catalog_create_impl(xm_ctx* ctx, const xm_segment* segs, uint64_t n_segs,
+                                     bool on_device, xm_catalog** out) {
+    if (!ctx || !out || (n_segs && !segs)) return XM_EINVAL;
     *out = nullptr;
     std::lock_guard<std::mutex> lk(ctx->mu);
     xm_catalog* cat = new (std::nothrow) xm_catalog();
     if (!cat) return XM_ENOMEM;
     xm_obj* staged = nullptr;
     xm_status st = guarded(ctx, [&] {
-        if (!(zone_height_deg > 0.0) || !std::isfinite(zone_height_deg) || zone_height_deg > 180.0)
-            throw InvalidArg{"zone height must be in (0, 180] degrees"};
-        if (180.0 / zone_height_deg > double(1 << 24))
-            throw InvalidArg{"zone height too small (more than 2^24 zones)"};
-        if (n >= 0xffffffffull) throw InvalidArg{"a segment holds fewer than 2^32-1 records"};
-        if (n && !recs) throw InvalidArg{"null records"};
-        const xm_obj* d = recs;
-        if (!on_device && n) {
-            staged = static_cast<xm_obj*>(dalloc(ctx, n * sizeof(xm_obj)));
-            XM_HIP(hipMemcpyAsync(staged, recs, n * sizeof(xm_obj), hipMemcpyHostToDevice, ctx->stream));
-            d = staged;
+        if (n_segs == 0) throw InvalidArg{"a catalog needs at least one segment"};
+        if (n_segs > MAX_SEGMENTS) throw InvalidArg{"more than 65536 segments in one catalog"};
+        std::vector<const xm_obj*> recs(n_segs);
+        std::vector<uint64_t> ns(n_segs);
+        std::vector<double> hs(n_segs);
+        uint64_t total = 0;
+        for (uint64_t s = 0; s < n_segs; s++) {
+            recs[s] = segs[s].records;
+            ns[s] = segs[s].n;
+            hs[s] = segs[s].zone_height_deg;
+            if (ns[s] >= 0xffffffffull) throw InvalidArg{"a segment holds fewer than 2^32-1 records"};
+            total += ns[s];
+            if (total >= 0xffffffffull) throw InvalidArg{"a catalog holds fewer than 2^32-1 records"};
         }
-        catalog_build(ctx, d, n, zone_height_deg, cat);
+        if (!on_device && total) {
+            // one staging buffer for all segments, one copy each
+            staged = static_cast<xm_obj*>(dalloc(ctx, total * sizeof(xm_obj)));
+            uint64_t off = 0;
+            for (uint64_t s = 0; s < n_segs; s++) {
+                if (ns[s] && !recs[s]) throw InvalidArg{"null records"};
+                if (ns[s])
+                    XM_HIP(hipMemcpyAsync(staged + off, recs[s], ns[s] * sizeof(xm_obj),
+                                          hipMemcpyHostToDevice, ctx->stream));
+                recs[s] = ns[s] ? staged + off : nullptr;
+                off += ns[s];
+            }
+        }
+        catalog_build(ctx, recs, ns, hs, cat);
         if (staged) { dfree(ctx, staged); staged = nullptr; }
         sync(ctx);
     });
     if (st != XM_OK) {
         (void)guarded(ctx, [&] {
+            sync(ctx);
             if (staged) dfree(ctx, staged);
             catalog_free(cat);
         });
@@ -503,12 +700,24 @@ static xm_status catalog_create_impl(xm_ctx* ctx, const xm_obj* recs, bool on_de
 
 xm_status xm_catalog_create(xm_ctx* ctx, const xm_obj* records, uint64_t n, double zone_height_deg,
                             xm_catalog** out) {
-    return catalog_create_impl(ctx, records, false, n, zone_height_deg, out);
+    const xm_segment sg{records, n, zone_height_deg};
+    return catalog_create_impl(ctx, &sg, 1, false, out);
 }
 
 xm_status xm_catalog_create_device(xm_ctx* ctx, const xm_obj* d_records, uint64_t n,
                                    double zone_height_deg, xm_catalog** out) {
-    return catalog_create_impl(ctx, d_records, true, n, zone_height_deg, out);
+    const xm_segment sg{d_records, n, zone_height_deg};
+    return catalog_create_impl(ctx, &sg, 1, true, out);
+}
+
+xm_status xm_catalog_create_segments(xm_ctx* ctx, const xm_segment* segs, uint64_t n_segs,
+                                     xm_catalog** out) {
+    return catalog_create_impl(ctx, segs, n_segs, false, out);
+}
+
+xm_status xm_catalog_create_segments_device(xm_ctx* ctx, const xm_segment* segs, uint64_t n_segs,
+                                            xm_catalog** out) {
+    return catalog_create_impl(ctx, segs, n_segs, true, out);
 }
 
 void xm_catalog_destroy(xm_catalog* cat) {
@@ -522,11 +731,34 @@ void xm_catalog_destroy(xm_catalog* cat) {
 
 xm_status xm_catalog_info(const xm_catalog* cat, uint64_t* n, double* zone_height_deg,
                           int32_t* zmin, int32_t* zmax) {
-    if (!cat) return XM_EINVAL;
+    if (!cat || cat->seg.empty()) return XM_EINVAL;
+    int32_t lo = INT_MAX, hi = -1;
+    for (const auto& d : cat->seg)
+        if (d.n) { lo = std::min(lo, d.zmin); hi = std::max(hi, d.zmax); }
+    if (hi < 0) lo = 0;
     if (n) *n = cat->n;
-    if (zone_height_deg) *zone_height_deg = cat->h_deg;
-    if (zmin) *zmin = cat->zmin;
-    if (zmax) *zmax = cat->zmax;
+    if (zone_height_deg) *zone_height_deg = cat->seg[0].h_deg;
+    if (zmin) *zmin = lo;
+    if (zmax) *zmax = hi;
+    return XM_OK;
+}
+
+xm_status xm_catalog_segment_count(const xm_catalog* cat, uint64_t* n_segs) {
+    if (!cat || !n_segs) return XM_EINVAL;
+    *n_segs = cat->seg.size();
+    return XM_OK;
+}
+
+xm_status xm_catalog_segment_info(const xm_catalog* cat, uint64_t seg, uint64_t* first_row,
+                                  uint64_t* n, double* zone_height_deg, int32_t* zmin,
+                                  int32_t* zmax) {
+    if (!cat || seg >= cat->seg.size()) return XM_EINVAL;
+    const SegDesc& d = cat->seg[seg];
+    if (first_row) *first_row = d.row_base;
+    if (n) *n = d.n;
+    if (zone_height_deg) *zone_height_deg = d.h_deg;
+    if (zmin) *zmin = d.n ? d.zmin : 0;
+    if (zmax) *zmax = d.n ? d.zmax : -1;
     return XM_OK;
 }
 
@@ -556,28 +788,35 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
     });
 }
 
-xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double radius_deg,
-                   xm_precision precision, uint64_t capacity_hint, xm_result** out,
-                   xm_match_stats* stats) {
-    if (!ctx || !out) return XM_EINVAL;
+}  // extern "C"
+
+namespace {
+
+// The join of a batch of jobs, with overflow-and-retry.  batch = keep the job
+// column and the per-job counts in the result.
+xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, const xm_job* jobs,
+                     uint64_t n_jobs, xm_precision precision, uint64_t capacity_hint, bool batch,
+                     xm_result** out, xm_match_stats* stats) {
+    if (!ctx || !out || !a || !b || (n_jobs && !jobs)) return XM_EINVAL;
     *out = nullptr;
     std::lock_guard<std::mutex> lk(ctx->mu);
     xm_result* res = new (std::nothrow) xm_result();
     if (!res) return XM_ENOMEM;
     res->ctx = ctx;
     RegionOut ro{};
-    uint64_t* scratch = nullptr;  // prefix[n_regions] + tail[2]
+    JobTable jt;
+    uint64_t* scratch = nullptr;  // prefix[n_regions] + tail[4]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     xm_match_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
     xm_status st = guarded(ctx, [&] {
-        check_join_args(a, b, radius_deg);
+        if (a->ctx != ctx || b->ctx != ctx) throw InvalidArg{"catalog belongs to another context"};
         if (precision != XM_FP64_PLAIN && precision != XM_FP32_FILTER_FP64_RECHECK)
             throw InvalidArg{"unknown precision mode"};
         const bool f32 = precision == XM_FP32_FILTER_FP64_RECHECK;
         xm_catalog* A = const_cast<xm_catalog*>(a);
         xm_catalog* Bm = const_cast<xm_catalog*>(b);
-        ensure_chunks(ctx, A);
+        make_jobs(ctx, A, b, jobs, n_jobs, jt);
         if (f32) {
             for (xm_catalog* c : {A, Bm}) {
                 if (c->row32 || c->n == 0) continue;
@@ -586,8 +825,12 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
                                 static_cast<Row32*>(c->row32));
             }
         }
-        const PairParams pp = make_params(ctx, A, b, radius_deg);
-        uint64_t cap = capacity_hint ? capacity_hint : 2 * std::max(a->n, b->n);
+        uint64_t cap = capacity_hint;
+        if (!cap) {
+            // the reference's default, 2 * max(nA, nB) per job (Worker.cu:295-300)
+            for (uint64_t j = 0; j < n_jobs; j++)
+                cap += 2 * std::max<uint64_t>(A->seg[jobs[j].seg_a].n, b->seg[jobs[j].seg_b].n);
+        }
         cap = std::max<uint64_t>(cap, 1024);
         XM_HIP(hipEventCreate(&ev0));
         XM_HIP(hipEventCreate(&ev1));
@@ -595,11 +838,12 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
         // a one-block scan sizes them and k_compact packs them into the result; the
         // only host round trip reads the total and the fullest region.
         ro.n_regions = std::max<uint32_t>(
-            1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(uint32_t(std::max<uint64_t>(A->max_chunks, 1)))));
+            1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(std::max<uint64_t>(jt.max_waves, 1))));
         ro.region_cap = std::max<uint64_t>(1, (cap + ro.n_regions - 1) / ro.n_regions);
         ro.counters = static_cast<uint64_t*>(
             dalloc(ctx, size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t)));
         scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(ro.n_regions) + 4) * sizeof(uint64_t)));
+        if (batch) ro.job_count = static_cast<unsigned long long*>(dalloc(ctx, n_jobs * 8));
         uint64_t* prefix = scratch;
         uint64_t* tail = scratch + ro.n_regions;
         uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
@@ -611,26 +855,30 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             res->id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
             res->id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
             res->dist = static_cast<double*>(dalloc(ctx, slots * 8));
+            if (batch) {
+                ro.job = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
+                res->job = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
+                launch_fill_u64(ctx, reinterpret_cast<uint64_t*>(ro.job_count), n_jobs, 0);
+            }
             res->cap = slots;
             launch_fill_u64(ctx, ro.counters, size_t(ro.n_regions) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev0, ctx->stream));
-            launch_pair(ctx, A, b, pp, f32, ro);
+            launch_pair(ctx, A, b, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
             launch_region_scan(ctx, ro, prefix, tail);
-            launch_compact(ctx, ro, prefix, res->id_a, res->id_b, res->dist);
+            launch_compact(ctx, ro, prefix, res->id_a, res->id_b, res->dist, res->job);
             XM_HIP(hipEventRecord(ev1, ctx->stream));
             XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
-            if (A->n_chunks_dev)
-                XM_HIP(hipMemcpyAsync(pin + 4, A->n_chunks_dev, sizeof(uint32_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
-            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
-            ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr;
+            XM_HIP(hipMemcpyAsync(pin + 4, jt.prefix + jt.n_jobs, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2); dfree(ctx, ro.job);
+            ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr; ro.job = nullptr;
             sync(ctx);
             float ms = 0.f;
             XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
             st_local.join_ms += ms;
             const uint64_t total = pin[0], most = pin[1];
-            st_local.n_chunks = A->n_chunks_dev ? reinterpret_cast<uint32_t*>(pin + 4)[0] : 0;
+            st_local.n_chunks = reinterpret_cast<uint32_t*>(pin + 4)[0];
             if (most <= ro.region_cap) {
                 res->n = total;
                 st_local.band_hits = pin[2];
@@ -642,17 +890,27 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
             // overflow: every counter holds its region's exact total; re-run with
             // regions that fit the fullest one
             st_local.n_retries++;
-            dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist);
-            res->id_a = nullptr; res->id_b = nullptr; res->dist = nullptr;
+            dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist); dfree(ctx, res->job);
+            res->id_a = nullptr; res->id_b = nullptr; res->dist = nullptr; res->job = nullptr;
             ro.region_cap = most;
             if (attempt > 2) throw std::runtime_error("result size did not converge");
+        }
+        if (batch) {
+            res->job_counts.resize(n_jobs);
+            XM_HIP(hipMemcpyAsync(res->job_counts.data(), ro.job_count, n_jobs * 8,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
         }
         st_local.n_matches = res->n;
     });
     (void)guarded(ctx, [&] {
+        sync(ctx);
         dfree(ctx, ro.counters);
         dfree(ctx, scratch);
-        dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
+        dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2); dfree(ctx, ro.job);
+        dfree(ctx, reinterpret_cast<void*>(ro.job_count));
+        dfree(ctx, jt.jobs);
+        dfree(ctx, jt.prefix);
         if (ev0) XM_HIP(hipEventDestroy(ev0));
         if (ev1) XM_HIP(hipEventDestroy(ev1));
     });
@@ -660,11 +918,60 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     if (st != XM_OK) {
         (void)guarded(ctx, [&] {
             dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist);
+            dfree(ctx, res->job);
         });
         delete res;
         return st;
     }
     *out = res;
+    return XM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double radius_deg,
+                   xm_precision precision, uint64_t capacity_hint, xm_result** out,
+                   xm_match_stats* stats) {
+    if (!ctx || !out) return XM_EINVAL;
+    *out = nullptr;
+    if (!a || !b) {
+        ctx->last_error = "null catalog";
+        return XM_EINVAL;
+    }
+    if (a->seg.size() != 1 || b->seg.size() != 1) {
+        ctx->last_error = "xm_match joins one-segment catalogs; use xm_match_batch";
+        return XM_EINVAL;
+    }
+    const xm_job job{0, 0, radius_deg};
+    return match_jobs(ctx, a, b, &job, 1, precision, capacity_hint, false, out, stats);
+}
+
+xm_status xm_match_batch(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, const xm_job* jobs,
+                         uint64_t n_jobs, xm_precision precision, uint64_t capacity_hint,
+                         xm_result** out, xm_match_stats* stats) {
+    return match_jobs(ctx, a, b, jobs, n_jobs, precision, capacity_hint, true, out, stats);
+}
+
+xm_status xm_result_jobs(xm_ctx* ctx, const xm_result* res, uint32_t* job, uint64_t* job_counts) {
+    if (!ctx || !res) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        if (!res->job && res->job_counts.empty())
+            throw InvalidArg{"not the result of a batch join"};
+        if (job && res->n) {
+            XM_HIP(hipMemcpyAsync(job, res->job, res->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
+        }
+        if (job_counts)
+            std::memcpy(job_counts, res->job_counts.data(), res->job_counts.size() * 8);
+    });
+}
+
+xm_status xm_result_job_device_ptr(const xm_result* res, const uint32_t** job) {
+    if (!res || !job) return XM_EINVAL;
+    *job = res->job;
     return XM_OK;
 }
 
@@ -706,6 +1013,7 @@ void xm_result_destroy(xm_result* res) {
             dfree(res->ctx, res->id_a);
             dfree(res->ctx, res->id_b);
             dfree(res->ctx, res->dist);
+            dfree(res->ctx, res->job);
         });
     }
     delete res;

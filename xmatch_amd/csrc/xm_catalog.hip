@@ -8,6 +8,12 @@
 //                                                  FP64 unit vector, radec2xyz of
 //                                                  Worker.cu:31-40 done once per source
 //                                                  instead of per pair, and the id)
+//
+// Many segments (the reference sorts each Segment on its own, one Sorter::Sort
+// per segment, BoostXmatch.cpp:250-263) are prepared together: k_seg_range finds
+// every segment's zone range, k_seg_words packs segment | zone | ra | row words in
+// a per-segment layout, one radix sort orders all of them, and k_gather_seg /
+// k_zone_offsets split them back into per-segment zone tables.
 #include <algorithm>
 
 #include "xm_internal.h"
@@ -77,6 +83,128 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
     }
 }
 
+// ---- many segments: a block takes a tile of SEG_TILE consecutive rows of the
+// concatenation and walks the segments that overlap it, so every segment-wide
+// quantity costs one atomic per (block, segment).
+constexpr uint32_t SEG_TILE = TPB * 16;
+
+// Last segment whose row_base <= row (segments are in row order; empty segments
+// share their base with the next one, so the search lands on a non-empty one).
+__device__ inline uint32_t seg_of_row(const SegDesc* __restrict__ seg, uint32_t n_seg, uint64_t row) {
+    uint32_t lo = 0, hi = n_seg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (seg[mid].row_base <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int32_t block_min(int32_t v, int32_t* s) {
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) s[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    v = s[0];
+    for (int q = 1; q < TPB / WAVE; q++) v = min(v, s[q]);
+    __syncthreads();
+    return v;
+}
+
+__device__ inline int32_t block_max(int32_t v, int32_t* s) {
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) s[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    v = s[0];
+    for (int q = 1; q < TPB / WAVE; q++) v = max(v, s[q]);
+    __syncthreads();
+    return v;
+}
+
+__global__ __launch_bounds__(TPB) void k_seg_range(const SegDesc* __restrict__ seg, uint32_t n_seg,
+                                                   uint64_t n, int32_t* __restrict__ range) {
+    __shared__ int32_t s_red[TPB / WAVE];
+    const uint64_t t0 = uint64_t(blockIdx.x) * SEG_TILE;
+    const uint64_t t1 = t0 + SEG_TILE < n ? t0 + SEG_TILE : n;
+    for (uint32_t s = seg_of_row(seg, n_seg, t0); s < n_seg; s++) {
+        const uint64_t base = seg[s].row_base, end = base + seg[s].n;
+        if (base >= t1) break;
+        if (end <= t0) continue;
+        const uint64_t lo = base > t0 ? base : t0, hi = end < t1 ? end : t1;
+        const xm_obj* __restrict__ recs = seg[s].recs;
+        const double h = seg[s].h_deg;
+        const int32_t nzs = seg[s].n_zones;
+        int32_t zmn = 0x7fffffff, zmx = -1, nbad = 0;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += TPB) {
+            const double ra = recs[i - base].ra_deg;
+            const double dec = recs[i - base].dec_deg;
+            if (valid_record(ra, dec)) {
+                const int32_t z = zone_of(dec, h, nzs);
+                zmn = min(zmn, z);
+                zmx = max(zmx, z);
+            } else {
+                nbad++;
+            }
+        }
+        zmn = block_min(zmn, s_red);
+        zmx = block_max(zmx, s_red);
+        nbad = block_max(nbad > 0 ? 1 : 0, s_red);
+        if (threadIdx.x == 0) {
+            if (zmx >= 0) {
+                atomicMin(&range[3 * s], zmn);
+                atomicMax(&range[3 * s + 1], zmx);
+            }
+            if (nbad) atomicAdd(&range[3 * s + 2], 1);
+        }
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ seg, uint32_t n_seg,
+                                                   uint64_t n, int seg_shift,
+                                                   uint64_t* __restrict__ words) {
+    const uint64_t t0 = uint64_t(blockIdx.x) * SEG_TILE;
+    const uint64_t t1 = t0 + SEG_TILE < n ? t0 + SEG_TILE : n;
+    for (uint32_t s = seg_of_row(seg, n_seg, t0); s < n_seg; s++) {
+        const uint64_t base = seg[s].row_base, end = base + seg[s].n;
+        if (base >= t1) break;
+        if (end <= t0) continue;
+        const uint64_t lo = base > t0 ? base : t0, hi = end < t1 ? end : t1;
+        const xm_obj* __restrict__ recs = seg[s].recs;
+        const double h = seg[s].h_deg;
+        const int32_t nzs = seg[s].n_zones, zb = seg[s].zone_base;
+        const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
+        const uint64_t top = seg_shift < 64 ? uint64_t(s) << seg_shift : 0;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += TPB) {
+            const double ra = recs[i - base].ra_deg;
+            const double dec = recs[i - base].dec_deg;
+            const int32_t z = zone_of(dec, h, nzs) - zb;
+            const uint64_t q = ra_quantum(ra) >> (RA_BITS - rb);
+            words[i] = top | (uint64_t(uint32_t(z)) << (rb + ib)) | (q << ib) | (i - base);
+        }
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
+                                                    uint64_t* __restrict__ words, uint32_t n,
+                                                    Row64* __restrict__ rows) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t wd = words[i];
+    const uint32_t s = seg_shift < 64 ? uint32_t(wd >> seg_shift) : 0u;
+    const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
+    const uint64_t loc = wd & ((uint64_t(1) << ib) - 1);
+    const uint64_t ra_q = (wd >> ib) & ((uint64_t(1) << rb) - 1);
+    const uint64_t zrel = (wd >> (ib + rb)) & ((uint64_t(1) << seg[s].zone_bits) - 1);
+    const xm_obj rec = seg[s].recs[loc];
+    Row64 r;
+    unit_vector(rec.ra_deg, rec.dec_deg, &r.x, &r.y, &r.z);
+    r.id = rec.id;
+    rows[i] = r;
+    const uint64_t zone = uint64_t(seg[s].zone_base) + zrel;
+    words[i] = (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb));
+}
+
 __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
                                                 uint64_t* __restrict__ words, uint32_t n,
                                                 PackLayout pl, Row64* __restrict__ rows) {
@@ -96,41 +224,55 @@ __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
     words[i] = (zone << RA_BITS) | (ra_q << (RA_BITS - pl.ra_bits));
 }
 
-// zoff[z - zmin] = first sorted row whose zone is >= z, for z in [zmin, zmax+1].
-// Row i writes the entries of the zones in (zone(i-1), zone(i)].
+// zoff[zoff_base(s) + z - zmin(s)] = first row of segment s whose zone is >= z,
+// for z in [zmin(s), zmax(s)]; the entry after a segment's last is the next
+// segment's first row, and zoff[n_zoff - 1] = n.  Row i writes the entries of the
+// zones in (zone(i-1), zone(i)] of its segment (from zmin(s) at a segment start).
 __global__ __launch_bounds__(TPB) void k_zone_offsets(const uint64_t* __restrict__ keys, uint32_t n,
-                                                      int32_t zmin, int32_t zmax,
-                                                      uint32_t* __restrict__ zoff) {
+                                                      const SegDesc* __restrict__ seg,
+                                                      uint32_t n_zoff, uint32_t* __restrict__ zoff) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i > n) return;
-    const int32_t zp = i == 0 ? zmin - 1 : key_zone(keys[i - 1]);
-    const int32_t zc = i == n ? zmax + 1 : key_zone(keys[i]);
-    for (int32_t z = zp + 1; z <= zc; z++) zoff[z - zmin] = i;
+    if (i == n) {
+        zoff[n_zoff - 1] = n;
+        return;
+    }
+    const uint64_t k = keys[i];
+    const uint32_t s = key_seg(k);
+    const int32_t zmin = seg[s].zmin;
+    const uint32_t base = seg[s].zoff_base;
+    int32_t zp = zmin - 1;
+    if (i > 0 && key_seg(keys[i - 1]) == s) zp = key_zone(keys[i - 1]);
+    const int32_t zc = key_zone(k);
+    for (int32_t z = zp + 1; z <= zc; z++) zoff[base + uint32_t(z - zmin)] = i;
 }
 
 __global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict__ zoff,
-                                                      int32_t nzone,
+                                                      uint32_t n_entries,
                                                       uint32_t* __restrict__ counts) {
-    const int32_t k = blockIdx.x * TPB + threadIdx.x;
-    if (k >= nzone) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= n_entries) return;
     const uint32_t rows = zoff[k + 1] - zoff[k];
     counts[k] = (rows + CHUNK_ROWS - 1) / CHUNK_ROWS;
 }
 
 __global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__ zoff,
                                                     const uint32_t* __restrict__ cstart,
-                                                    int32_t zmin, int32_t nzone,
+                                                    const uint64_t* __restrict__ keys,
+                                                    uint32_t n_entries,
                                                     Chunk* __restrict__ chunks) {
-    const int32_t k = blockIdx.x * TPB + threadIdx.x;
-    if (k >= nzone) return;
+    const uint32_t k = blockIdx.x * TPB + threadIdx.x;
+    if (k >= n_entries) return;
     const uint32_t r0 = zoff[k], r1 = zoff[k + 1];
+    if (r1 <= r0) return;
+    const int32_t zone = key_zone(keys[r0]);
     uint32_t c = cstart[k];
     for (uint32_t r = r0; r < r1; r += CHUNK_ROWS, c++) {
         Chunk ch;
         ch.start = r;
         ch.count = (r1 - r) < uint32_t(CHUNK_ROWS) ? (r1 - r) : uint32_t(CHUNK_ROWS);
-        ch.zone = zmin + k;
-        ch.aq = 0;
+        ch.zone = zone;
+        ch.pad = 0;
         chunks[c] = ch;
     }
 }
@@ -164,28 +306,54 @@ void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n,
     });
 }
 
-void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin, int32_t zmax,
-                         uint32_t* zoff) {
+void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range) {
+    if (n == 0 || n_seg == 0) return;
+    launch_timed(ctx, "seg_range", [&] {
+        hipLaunchKernelGGL(k_seg_range, dim3(uint32_t((n + SEG_TILE - 1) / SEG_TILE)), dim3(TPB), 0,
+                           ctx->stream, seg, n_seg, n, range);
+    });
+}
+
+void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
+                      uint64_t* words) {
+    if (n == 0 || n_seg == 0) return;
+    launch_timed(ctx, "make_keys", [&] {
+        hipLaunchKernelGGL(k_seg_words, dim3(uint32_t((n + SEG_TILE - 1) / SEG_TILE)), dim3(TPB), 0,
+                           ctx->stream, seg, n_seg, n, seg_shift, words);
+    });
+}
+
+void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, uint64_t* words, uint64_t n,
+                       Row64* rows) {
+    if (n == 0) return;
+    launch_timed(ctx, "gather", [&] {
+        hipLaunchKernelGGL(k_gather_seg, grid_for(n), dim3(TPB), 0, ctx->stream, seg, seg_shift,
+                           words, uint32_t(n), rows);
+    });
+}
+
+void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, const SegDesc* seg,
+                         uint64_t n_zoff, uint32_t* zoff) {
     launch_timed(ctx, "zone_offsets", [&] {
         hipLaunchKernelGGL(k_zone_offsets, grid_for(n + 1), dim3(TPB), 0, ctx->stream, keys,
-                           uint32_t(n), zmin, zmax, zoff);
+                           uint32_t(n), seg, uint32_t(n_zoff), zoff);
     });
 }
 
-void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts) {
-    if (nzone <= 0) return;
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries, uint32_t* counts) {
+    if (n_entries == 0) return;
     launch_timed(ctx, "chunk_counts", [&] {
-        hipLaunchKernelGGL(k_chunk_counts, grid_for(uint64_t(nzone)), dim3(TPB), 0, ctx->stream,
-                           zoff, nzone, counts);
+        hipLaunchKernelGGL(k_chunk_counts, grid_for(n_entries), dim3(TPB), 0, ctx->stream, zoff,
+                           uint32_t(n_entries), counts);
     });
 }
 
-void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart, int32_t zmin,
-                       int32_t nzone, Chunk* chunks) {
-    if (nzone <= 0) return;
+void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
+                       const uint64_t* keys, uint64_t n_entries, Chunk* chunks) {
+    if (n_entries == 0) return;
     launch_timed(ctx, "chunk_fill", [&] {
-        hipLaunchKernelGGL(k_chunk_fill, grid_for(uint64_t(nzone)), dim3(TPB), 0, ctx->stream,
-                           zoff, cstart, zmin, nzone, chunks);
+        hipLaunchKernelGGL(k_chunk_fill, grid_for(n_entries), dim3(TPB), 0, ctx->stream, zoff,
+                           cstart, keys, uint32_t(n_entries), chunks);
     });
 }
 

@@ -116,7 +116,15 @@ XM_HD uint64_t make_key(int32_t zone, double ra_deg) {
     return (uint64_t(uint32_t(zone)) << RA_BITS) | ra_quantum(ra_deg);
 }
 
-XM_HD int32_t key_zone(uint64_t key) { return int32_t(key >> RA_BITS); }
+// A catalog key is seg << 48 | zone << 24 | ra quantum: the segment of the row
+// (0 for a one-segment catalog), its zone (< 2^24) and its ra.  Keys of one
+// segment are sorted; the segment bits keep the segments' key ranges apart.
+constexpr int KEY_SEG_SHIFT = 48;
+constexpr uint64_t MAX_SEGMENTS = uint64_t(1) << (64 - KEY_SEG_SHIFT);
+constexpr uint64_t ZONE_MASK = (uint64_t(1) << (KEY_SEG_SHIFT - RA_BITS)) - 1;
+
+XM_HD int32_t key_zone(uint64_t key) { return int32_t((key >> RA_BITS) & ZONE_MASK); }
+XM_HD uint32_t key_seg(uint64_t key) { return uint32_t(key >> KEY_SEG_SHIFT); }
 
 // Chord threshold (2 sin(r/2))^2 of Worker.cu:315-317 with the sine above.
 XM_HD double chord2_of_radius(double sr_deg) {

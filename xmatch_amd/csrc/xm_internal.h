@@ -60,23 +60,43 @@ struct xm_ctx {
     int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
 };
 
+namespace xm {
+
+// One segment of a catalog: a contiguous run of rows, zone-sorted on its own and
+// indexed by its own zone table.  A catalog made by xm_catalog_create has one
+// segment; xm_catalog_create_segments puts many in one set of arrays, built by one
+// sort, so that a batch of joins runs as one launch.
+struct SegDesc {
+    const xm_obj* recs;   // records during the build (device), null afterwards
+    uint32_t row_base;    // first row in the catalog arrays
+    uint32_t n;           // rows
+    int32_t zmin, zmax;   // occupied zone range (zmin > zmax when empty)
+    uint32_t zoff_base;   // zoff index of zone zmin; entries zoff_base .. + (zmax-zmin+1)
+    int32_t n_zones;      // ceil(180 / h)
+    double h_deg;         // zone height, degrees
+    int64_t ra_gran;      // ra granularity of the keys, in 2^-24 turn quanta
+    // build word layout: zone - zone_base in zone_bits above ra_bits above idx_bits
+    int32_t zone_base;
+    int32_t zone_bits, ra_bits, idx_bits;
+};
+
+}  // namespace xm
+
 struct xm_catalog {
     xm_ctx* ctx = nullptr;
-    uint64_t n = 0;
-    double h_deg = 0;
-    int32_t n_zones = 0;
-    int32_t zmin = 0, zmax = -1;
-    int64_t ra_gran = 1;         // ra granularity of the keys, in 2^-24 turn quanta
-    uint64_t* key = nullptr;     // sorted (zone << RA_BITS | ra quantum)
-    void* row = nullptr;         // xm::Row64[n]: FP64 unit vector + id, sorted order
-    void* row32 = nullptr;       // xm::Row32[n]: FP32 unit vectors, made on first FP32 join
-    uint32_t* zoff = nullptr;    // [zmax-zmin+2]: first row of zones zmin..zmax+1
+    uint64_t n = 0;                  // rows of all segments
+    std::vector<xm::SegDesc> seg;    // host copy of the segment table
+    xm::SegDesc* seg_dev = nullptr;  // device copy
+    uint64_t* key = nullptr;         // sorted catalog keys (seg << 48 | zone << 24 | ra)
+    void* row = nullptr;             // xm::Row64[n]: FP64 unit vector + id, sorted order
+    void* row32 = nullptr;           // xm::Row32[n]: FP32 unit vectors, made on first FP32 join
+    uint32_t* zoff = nullptr;        // [n_zoff]: first row of each segment zone, then n
+    uint64_t n_zoff = 0;             // sum over segments of (zmax - zmin + 1), plus 1
     // A-side row chunks (built on first use as the A catalog of a join): the
     // table is sized for max_chunks, the true count stays on the device
     void* chunks = nullptr;
     uint64_t max_chunks = 0;
-    uint32_t* chunk_start = nullptr;   // [zmax-zmin+2]: exclusive scan of chunks per zone
-    const uint32_t* n_chunks_dev = nullptr;  // = chunk_start + (zmax-zmin+1)
+    uint32_t* chunk_start = nullptr;   // [n_zoff]: exclusive scan of chunks per zoff entry
 };
 
 struct xm_result {
@@ -86,6 +106,8 @@ struct xm_result {
     int64_t* id_a = nullptr;
     int64_t* id_b = nullptr;
     double* dist = nullptr;
+    uint32_t* job = nullptr;            // batch joins: job index of every pair
+    std::vector<uint64_t> job_counts;   // batch joins: pairs per job
 };
 
 namespace xm {
@@ -106,13 +128,12 @@ struct Row32 {
     uint32_t pad;
 };
 
-// A run of at most CHUNK_ROWS consecutive A rows of one zone.  aq is the ra
-// half-width of the zone's window in quanta (alpha_quanta), filled per join.
+// A run of at most CHUNK_ROWS consecutive A rows of one zone of one segment.
 struct Chunk {
     uint32_t start;
     uint32_t count;
     int32_t zone;
-    int32_t aq;
+    int32_t pad;
 };
 
 // ---- memory: the context's cache of device blocks.  Every kernel of a context
@@ -147,18 +168,31 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
                      int hi_bit, uint64_t skip_mask_digits);
 
 // ---- catalog kernels (xm_catalog.hip)
-// Packed sort words (pack_word) of the records, their zone range and bad count.
+// One-segment build: packed sort words (pack_word) of the records, their zone
+// range and bad count, in one pass.
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad);
+// Many-segment build, pass 1: zone range and bad count of every segment
+// (range[3 s] = zmin, range[3 s + 1] = zmax, range[3 s + 2] = bad).
+void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range);
+// Pass 2: words = s << seg_shift | (zone - zone_base) | ra | row within the segment,
+// in the segment's layout.
+void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
+                      uint64_t* words);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
-// carries) and, in place, words[i] -> catalog key zone << 24 | ra << (24 - ra_bits).
+// carries) and, in place, words[i] -> catalog key.  The one-segment form takes
+// the layout by value; the many-segment form reads it from the segment table.
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n, PackLayout pl,
                    Row64* rows);
-void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, int32_t zmin,
-                         int32_t zmax, uint32_t* zoff);
-void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, int32_t nzone, uint32_t* counts);
-void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart, int32_t zmin,
-                       int32_t nzone, Chunk* chunks);
+void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, uint64_t* words, uint64_t n,
+                       Row64* rows);
+// zoff of every segment (see xm_catalog) from the sorted catalog keys.
+void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, const SegDesc* seg,
+                         uint64_t n_zoff, uint32_t* zoff);
+// counts[k] = chunks of zoff entry k (k < n_entries); chunks from the scanned counts.
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries, uint32_t* counts);
+void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
+                       const uint64_t* keys, uint64_t n_entries, Chunk* chunks);
 void launch_fill_u32(xm_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 void launch_fill_u64(xm_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v);
 void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
@@ -188,19 +222,40 @@ struct RegionOut {
     int64_t* id_a;   // [n_regions * region_cap]
     int64_t* id_b;
     double* d2;      // chord^2 of each hit
+    uint32_t* job;   // batch joins: job of each hit (else null)
+    unsigned long long* job_count;  // batch joins: hits per job (else null)
 };
-uint32_t pair_blocks(uint32_t n_chunks);
-// f32 = FP32 filter + FP64 recheck (needs A->row32 and B->row32), else FP64 plain.
-void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                 bool f32, const RegionOut& ro);
+// One join of a batch: A segment x B segment at one radius.  The host fills
+// everything but a_chunk_base; launch_job_prep fills that and the wave counts.
+struct JobDesc {
+    PairParams pp;
+    uint64_t b_key_prefix;     // B segment << 48
+    uint32_t a_zoff_lo;        // A segment's zoff entries [a_zoff_lo, a_zoff_hi)
+    uint32_t a_zoff_hi;
+    uint32_t b_zoff_base;      // zoff index of B zone b_zmin
+    int32_t b_zmin, b_zmax;
+    uint32_t a_chunk_base;     // first chunk of the A segment (device-filled)
+};
+// Per job: a_chunk_base = cstart[a_zoff_lo], waves[j] = its chunk count.
+void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t* cstart,
+                     uint32_t* waves);
+uint32_t pair_blocks(uint64_t n_waves);
+// One launch over every (job, A chunk): wave g belongs to the job j with
+// job_prefix[j] <= g < job_prefix[j+1] (job_prefix[n_jobs] = total, on the
+// device); the grid covers max_waves.  f32 = FP32 filter + FP64 recheck (needs
+// row32 of both catalogs), else FP64 plain.
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const JobDesc* jobs,
+                 const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
+                 const RegionOut& ro);
 void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
 // prefix[r] = hits of the regions before r (capped at region_cap each);
 // tail[0] = total hits, tail[1] = largest region count, tail[2] = FP32 band hits,
 // tail[3] = largest |chord_32 - chord_64| of a kept pair (double bits).
 void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail);
 // Packs the regions into contiguous arrays (region r's hits at prefix[r]) and
-// converts chord^2 to the FP64 distance in arcsec.
+// converts chord^2 to the FP64 distance in arcsec; job (may be null) gets the
+// job column of batch joins.
 void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
-                    int64_t* id_b, double* dist);
+                    int64_t* id_b, double* dist, uint32_t* job);
 
 }  // namespace xm

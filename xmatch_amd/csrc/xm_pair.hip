@@ -24,6 +24,12 @@
 // chord error; every kept pair is re-evaluated in FP64 exactly as above when it is
 // written, so the set and the distances are those of the FP64 form.
 //
+// A launch joins a batch of jobs (A segment x B segment x radius, JobDesc): the
+// waves are numbered over the concatenation of every job's A chunks, and a wave
+// finds its job by binary search of the jobs' wave prefix on the device.  A plain
+// join is a batch of one job.  The reference ran one job at a time per GPU thread
+// (JobManager.cpp:5-41, Worker.cu:430-485), each as (2nz+1) launches per A zone.
+//
 // Hits are compacted with ballot + mbcnt into a per-wave LDS stage; a full stage
 // (or the end of the chunk) costs one global atomic on the 64-bit append counter
 // of the block's output region.  A join's ~10^6 wavefronts spread over up to 1024
@@ -126,15 +132,19 @@ struct Out {
     double* __restrict__ d2;
     double thr;                                // FP64 chord threshold
     double t_lo;                               // FP32 form: below it the filter is sure
+    uint32_t* __restrict__ job;                // batch joins: this region's job column
+    unsigned long long* __restrict__ job_count;  // batch joins: this job's hit counter
+    uint32_t jid;
 };
 
 __device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro,
-                               const PairParams& pp) {
+                               const PairParams& pp, uint32_t jid) {
     const uint32_t r = blockIdx.x % ro.n_regions;
     const uint64_t base = uint64_t(r) * ro.region_cap;
     return Out{arow, brow, (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE),
                ro.region_cap, ro.id_a + base, ro.id_b + base, ro.d2 + base, pp.chord2,
-               double(pp.t_lo32)};
+               double(pp.t_lo32), ro.job ? ro.job + base : nullptr,
+               ro.job_count ? ro.job_count + jid : nullptr, jid};
 }
 
 // Write the wave's staged hits to its region: one atomic per call.  In the FP32
@@ -178,7 +188,10 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
         total += uint32_t(__popcll(mk[k]));
     }
     unsigned long long base = 0;
-    if (lane == 0 && total) base = atomicAdd(o.counter, (unsigned long long)total);
+    if (lane == 0 && total) {
+        base = atomicAdd(o.counter, (unsigned long long)total);
+        if (o.job_count) atomicAdd(o.job_count, (unsigned long long)total);
+    }
     base = __shfl(base, 0, WAVE);
     uint32_t before = 0;
 #pragma unroll
@@ -189,6 +202,7 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
                 o.id_a[slot] = ia[k];
                 o.id_b[slot] = ib[k];
                 o.d2[slot] = dd[k];
+                if (o.job) o.job[slot] = o.jid;
             }
         }
         before += uint32_t(__popcll(mk[k]));
@@ -239,10 +253,12 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ co
                                                  const int64_t* __restrict__ ta,
                                                  const int64_t* __restrict__ tb,
                                                  const double* __restrict__ td,
+                                                 const uint32_t* __restrict__ tj,
                                                  const uint64_t* __restrict__ prefix,
                                                  int64_t* __restrict__ id_a,
                                                  int64_t* __restrict__ id_b,
-                                                 double* __restrict__ dist) {
+                                                 double* __restrict__ dist,
+                                                 uint32_t* __restrict__ job) {
     const uint32_t r = blockIdx.x;
     uint64_t n = counters[size_t(r) * REGION_STRIDE];
     n = n < region_cap ? n : region_cap;
@@ -252,6 +268,8 @@ __global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ co
         id_b[dst + i] = tb[src + i];
         dist[dst + i] = dist_arcsec(td[src + i]);
     }
+    if (job)
+        for (uint64_t i = threadIdx.x; i < n; i += 256) job[dst + i] = tj[src + i];
 }
 
 // --------------------------------------------------------------- sparse ----
@@ -352,17 +370,18 @@ __device__ inline void dense_range(uint32_t j0, uint32_t j1, typename Prec<F32>:
 }
 
 // ----------------------------------------------------------------- join ----
-// One wavefront per A chunk; the chunk count lives on the device (*n_chunks), the
-// grid covers an upper bound and the waves past the count return at once.
-// F32 selects the FP32 filter: tests read the catalogs' FP32 rows (arow_t/brow_t)
-// and keep pairs below t_hi32; the FP64 rows serve ids and the recheck.
+// One wavefront per (job, A chunk).  The wave total lives on the device
+// (job_prefix[n_jobs]); the grid covers an upper bound and the waves past the
+// total return at once.  F32 selects the FP32 filter: tests read the catalogs'
+// FP32 rows (arow_t/brow_t) and keep pairs below t_hi32; the FP64 rows serve ids
+// and the recheck.
 template <bool F32>
 __global__ __launch_bounds__(PK_THREADS) void k_pair(
-    const Chunk* __restrict__ chunks, const uint32_t* __restrict__ n_chunks,
-    const uint64_t* __restrict__ akey, const Row64* __restrict__ arow,
-    const typename Prec<F32>::Row* __restrict__ arow_t, const uint64_t* __restrict__ bkey,
-    const Row64* __restrict__ brow, const typename Prec<F32>::Row* __restrict__ brow_t,
-    const uint32_t* __restrict__ bzoff, int32_t bzmin, int32_t bzmax, PairParams pp,
+    const JobDesc* __restrict__ jobs, const uint32_t* __restrict__ job_prefix, uint32_t n_jobs,
+    const Chunk* __restrict__ chunks, const uint64_t* __restrict__ akey,
+    const Row64* __restrict__ arow, const typename Prec<F32>::Row* __restrict__ arow_t,
+    const uint64_t* __restrict__ bkey, const Row64* __restrict__ brow,
+    const typename Prec<F32>::Row* __restrict__ brow_t, const uint32_t* __restrict__ zoff_b,
     RegionOut ro) {
     using T = typename Prec<F32>::T;
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
@@ -375,8 +394,21 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
 
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
-    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
-    if (c >= *n_chunks) return;  // whole wavefront; the kernel has no block barrier
+    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x) * PK_WAVES + w;
+    if (g >= job_prefix[n_jobs]) return;  // whole wavefront; the kernel has no block barrier
+    // the job: last j with job_prefix[j] <= g (jobs without chunks are skipped over)
+    uint32_t jlo = 0, jhi = n_jobs - 1;
+    while (jlo < jhi) {
+        const uint32_t mid = (jlo + jhi + 1) / 2;
+        if (job_prefix[mid] <= g) jlo = mid;
+        else jhi = mid - 1;
+    }
+    const uint32_t jid = uni(jlo);
+    const PairParams pp = jobs[jid].pp;
+    const uint64_t bpre = jobs[jid].b_key_prefix;
+    const int32_t bzmin = jobs[jid].b_zmin, bzmax = jobs[jid].b_zmax;
+    const uint32_t* __restrict__ bzoff = zoff_b + jobs[jid].b_zoff_base;
+    const uint32_t c = jobs[jid].a_chunk_base + (g - job_prefix[jid]);
 
     const uint32_t start = uni(chunks[c].start);
     const uint32_t cnt = uni(chunks[c].count);
@@ -407,7 +439,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
 
     const SparseCtx<T> x{start, cnt, int32_t(aq), s_aq[w], s_ax[w], s_ay[w], s_az[w], thr};
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o = make_out(arow, brow, ro, pp);
+    const Out o = make_out(arow, brow, ro, pp, jid);
 
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
@@ -418,6 +450,10 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
         uint64_t lo[2], hi[2];
         int32_t shift[2], lq[2], hq[2];
         const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
+        for (int t = 0; t < ni; t++) {
+            lo[t] |= bpre;
+            hi[t] |= bpre;
+        }
         if (dense) {
             for (int t = 0; t < ni; t++) {
                 const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
@@ -504,6 +540,16 @@ __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict
     }
 }
 
+__global__ __launch_bounds__(256) void k_job_prep(JobDesc* __restrict__ jobs, uint32_t n_jobs,
+                                                  const uint32_t* __restrict__ cstart,
+                                                  uint32_t* __restrict__ waves) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_jobs) return;
+    const uint32_t c0 = cstart[jobs[j].a_zoff_lo], c1 = cstart[jobs[j].a_zoff_hi];
+    jobs[j].a_chunk_base = c0;
+    waves[j] = jobs[j].b_zmin <= jobs[j].b_zmax ? c1 - c0 : 0u;  // empty B: no work
+}
+
 // FP64 row -> FP32 row (round to nearest), for the FP32 filter.
 __global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, uint64_t n,
                                                   Row32* __restrict__ out) {
@@ -521,16 +567,26 @@ __global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, 
 }  // namespace
 
 void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
-                    int64_t* id_b, double* dist) {
+                    int64_t* id_b, double* dist, uint32_t* job) {
     launch_timed(ctx, "compact", [&] {
         hipLaunchKernelGGL(k_compact, dim3(ro.n_regions), dim3(256), 0, ctx->stream,
                            (const uint64_t*)ro.counters, ro.region_cap, (const int64_t*)ro.id_a,
-                           (const int64_t*)ro.id_b, (const double*)ro.d2, prefix, id_a, id_b,
-                           dist);
+                           (const int64_t*)ro.id_b, (const double*)ro.d2,
+                           (const uint32_t*)ro.job, prefix, id_a, id_b, dist,
+                           ro.job ? job : nullptr);
     });
 }
 
-uint32_t pair_blocks(uint32_t n_chunks) { return (n_chunks + PK_WAVES - 1) / PK_WAVES; }
+uint32_t pair_blocks(uint64_t n_waves) { return uint32_t((n_waves + PK_WAVES - 1) / PK_WAVES); }
+
+void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t* cstart,
+                     uint32_t* waves) {
+    if (n_jobs == 0) return;
+    launch_timed(ctx, "job_prep", [&] {
+        hipLaunchKernelGGL(k_job_prep, dim3((n_jobs + 255) / 256), dim3(256), 0, ctx->stream, jobs,
+                           n_jobs, cstart, waves);
+    });
+}
 
 void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail) {
     launch_timed(ctx, "region_scan", [&] {
@@ -547,25 +603,26 @@ void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out) {
     });
 }
 
-void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const PairParams& pp,
-                 bool f32, const RegionOut& ro) {
-    if (B->n == 0 || A->max_chunks == 0) return;
-    const dim3 grid(pair_blocks(uint32_t(A->max_chunks)));
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const JobDesc* jobs,
+                 const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
+                 const RegionOut& ro) {
+    if (B->n == 0 || A->n == 0 || n_jobs == 0 || max_waves == 0) return;
+    const dim3 grid(pair_blocks(max_waves));
     const Row64* arow = static_cast<const Row64*>(A->row);
     const Row64* brow = static_cast<const Row64*>(B->row);
+    const Chunk* chunks = static_cast<const Chunk*>(A->chunks);
     if (f32) {
         launch_timed(ctx, "pair_fp32", [&] {
-            hipLaunchKernelGGL(k_pair<true>, grid, dim3(PK_THREADS), 0, ctx->stream,
-                               (const Chunk*)A->chunks, (const uint32_t*)A->n_chunks_dev, A->key,
-                               arow, static_cast<const Row32*>(A->row32), B->key, brow,
-                               static_cast<const Row32*>(B->row32), B->zoff, B->zmin, B->zmax, pp,
-                               ro);
+            hipLaunchKernelGGL(k_pair<true>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
+                               job_prefix, n_jobs, chunks, A->key, arow,
+                               static_cast<const Row32*>(A->row32), B->key, brow,
+                               static_cast<const Row32*>(B->row32), B->zoff, ro);
         });
     } else {
         launch_timed(ctx, "pair_fp64", [&] {
-            hipLaunchKernelGGL(k_pair<false>, grid, dim3(PK_THREADS), 0, ctx->stream,
-                               (const Chunk*)A->chunks, (const uint32_t*)A->n_chunks_dev, A->key,
-                               arow, arow, B->key, brow, brow, B->zoff, B->zmin, B->zmax, pp, ro);
+            hipLaunchKernelGGL(k_pair<false>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
+                               job_prefix, n_jobs, chunks, A->key, arow, arow, B->key, brow, brow,
+                               B->zoff, ro);
         });
     }
 }

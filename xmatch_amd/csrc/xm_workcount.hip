@@ -36,33 +36,63 @@ uint64_t lower(const std::vector<uint64_t>& k, uint64_t lo, uint64_t hi, uint64_
     return uint64_t(std::lower_bound(k.begin() + lo, k.begin() + hi, t) - k.begin());
 }
 
+// One segment's rows on the host: keys without the segment bits, rows, and its
+// zone table as local row indices (zoff[z - zmin] for z in [zmin, zmax + 1]).
+struct HostSeg {
+    SegDesc d;
+    std::vector<uint64_t> key;
+    std::vector<Row64> row;
+    std::vector<uint32_t> zoff;
+};
+
+void fetch_seg(xm_ctx* ctx, const xm_catalog* c, uint32_t s, HostSeg& h) {
+    h.d = c->seg[s];
+    const uint64_t n = h.d.n;
+    h.key.resize(n);
+    h.row.resize(n);
+    const uint64_t nt = n ? uint64_t(h.d.zmax - h.d.zmin) + 2 : 1;
+    h.zoff.assign(nt, 0);
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) XM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    };
+    d2h(h.key.data(), c->key + h.d.row_base, n * 8);
+    d2h(h.row.data(), static_cast<const Row64*>(c->row) + h.d.row_base, n * sizeof(Row64));
+    if (n) d2h(h.zoff.data(), c->zoff + h.d.zoff_base, nt * 4);
+    XM_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t keep = (uint64_t(1) << KEY_SEG_SHIFT) - 1;
+    for (auto& k : h.key) k &= keep;
+    for (auto& z : h.zoff) z -= h.d.row_base;
+}
+
 }  // namespace
 
-extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b,
-                                    double radius_deg, uint64_t* w_ref, uint64_t* w_cand,
-                                    uint64_t* w_tile) {
-    if (!ctx || !a || !b) return XM_EINVAL;
+extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b,
+                                        const xm_job* job, uint64_t* w_ref, uint64_t* w_cand,
+                                        uint64_t* w_tile) {
+    if (!ctx || !a || !b || !job) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     try {
         XM_HIP(hipSetDevice(ctx->device));
-        if (!(radius_deg > 0.0) || a->h_deg != b->h_deg) {
-            ctx->last_error = "bad radius or zone heights differ";
+        const double radius_deg = job->radius_deg;
+        if (job->seg_a >= a->seg.size() || job->seg_b >= b->seg.size() || !(radius_deg > 0.0) ||
+            a->seg[job->seg_a].h_deg != b->seg[job->seg_b].h_deg) {
+            ctx->last_error = "bad segment, bad radius or zone heights differ";
             return XM_EINVAL;
         }
-        std::vector<uint64_t> ka(a->n), kb(b->n);
-        std::vector<Row64> ra_rows(a->n), rb_rows(b->n);
-        const int64_t nta = int64_t(a->zmax) - a->zmin + 2, ntb = int64_t(b->zmax) - b->zmin + 2;
-        std::vector<uint32_t> za(size_t(std::max<int64_t>(nta, 1))), zb(size_t(std::max<int64_t>(ntb, 1)));
-        auto d2h = [&](void* dst, const void* src, size_t bytes) {
-            if (bytes) XM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        };
-        d2h(ka.data(), a->key, a->n * 8);
-        d2h(kb.data(), b->key, b->n * 8);
-        d2h(ra_rows.data(), a->row, a->n * sizeof(Row64));
-        d2h(rb_rows.data(), b->row, b->n * sizeof(Row64));
-        d2h(za.data(), a->zoff, nta * 4);
-        d2h(zb.data(), b->zoff, ntb * 4);
-        XM_HIP(hipStreamSynchronize(ctx->stream));
+        HostSeg A, B;
+        fetch_seg(ctx, a, job->seg_a, A);
+        fetch_seg(ctx, b, job->seg_b, B);
+        const std::vector<uint64_t>& ka = A.key;
+        const std::vector<uint64_t>& kb = B.key;
+        const std::vector<uint32_t>& za = A.zoff;
+        const std::vector<uint32_t>& zb = B.zoff;
+        const std::vector<Row64>& ra_rows = A.row;
+        const std::vector<Row64>& rb_rows = B.row;
+        struct Info { uint64_t n; int32_t zmin, zmax, n_zones; double h_deg; int64_t ra_gran; };
+        const Info ai{A.d.n, A.d.zmin, A.d.zmax, A.d.n_zones, A.d.h_deg, A.d.ra_gran};
+        const Info bi{B.d.n, B.d.zmin, B.d.zmax, B.d.n_zones, B.d.h_deg, B.d.ra_gran};
+        const Info* a_ = &ai;
+        const Info* b_ = &bi;
 
         // exact ra (radians in [0, 2pi)) of every row, from its unit vector
         const double two_pi = 6.283185307179586476925286766559;
@@ -76,15 +106,15 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
         };
         const std::vector<double> raA = ra_of(ra_rows), raB = ra_of(rb_rows);
 
-        const double h = a->h_deg;
+        const double h = a_->h_deg;
         const int32_t nz = int32_t(std::ceil(radius_deg / h));
-        const int32_t n_zones = a->n_zones;
+        const int32_t n_zones = a_->n_zones;
         const double atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
-        const int64_t gran = std::max(a->ra_gran, b->ra_gran);
+        const int64_t gran = std::max(a_->ra_gran, b_->ra_gran);
         const int64_t full = int64_t(1) << RA_BITS;
 
-        std::atomic<int64_t> next{a->n ? a->zmin : 1};
-        const int64_t zlast = a->n ? a->zmax : 0;
+        std::atomic<int64_t> next{a_->n ? a_->zmin : 1};
+        const int64_t zlast = a_->n ? a_->zmax : 0;
         unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
         std::vector<uint64_t> r_ref(nth, 0), r_cand(nth, 0), r_tile(nth, 0);
         auto work = [&](unsigned t) {
@@ -92,14 +122,14 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
             for (;;) {
                 const int64_t z = next.fetch_add(1);
                 if (z > zlast) break;
-                const uint64_t a0 = za[z - a->zmin], a1 = za[z - a->zmin + 1];
+                const uint64_t a0 = za[z - a_->zmin], a1 = za[z - a_->zmin + 1];
                 if (a1 <= a0) continue;
                 const double al = ref_alpha_rad(radius_deg, h, int32_t(z));
                 const int64_t aq = join_quanta(radius_deg, h, int32_t(z), atan_sin_theta, gran);
                 for (int64_t z2 = std::max<int64_t>(0, z - nz);
                      z2 <= std::min<int64_t>(n_zones - 1, z + nz); z2++) {
-                    if (z2 < b->zmin || z2 > b->zmax) continue;
-                    const uint64_t b0 = zb[z2 - b->zmin], b1 = zb[z2 - b->zmin + 1];
+                    if (z2 < b_->zmin || z2 > b_->zmax) continue;
+                    const uint64_t b0 = zb[z2 - b_->zmin], b1 = zb[z2 - b_->zmin + 1];
                     if (b1 <= b0) continue;
                     s_ref += (a1 - a0) * (b1 - b0);
                     // w_cand: per A row, B rows within the unwidened alpha (cyclic, exact
@@ -169,4 +199,11 @@ extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_c
         ctx->last_error = e.what();
         return XM_EINTERNAL;
     }
+}
+
+extern "C" xm_status xm_work_counts(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b,
+                                    double radius_deg, uint64_t* w_ref, uint64_t* w_cand,
+                                    uint64_t* w_tile) {
+    const xm_job job{0, 0, radius_deg};
+    return xm_work_counts_job(ctx, a, b, &job, w_ref, w_cand, w_tile);
 }

@@ -130,6 +130,62 @@ def config_plane(seed=3, n_a=200_000_000, n_b=200_000_000):
     return a, b, 1.0 * ARCSEC
 
 
+def uniform_cap(rng, n, ra_c, dec_c, radius_deg):
+    """Uniform inside the spherical cap of radius_deg about (ra_c, dec_c): cos of the
+    angle from the centre uniform in [cos r, 1], position angle uniform."""
+    ct = 1.0 - rng.random(n) * (1.0 - np.cos(np.radians(radius_deg)))
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = 2.0 * np.pi * rng.random(n)
+    lx, ly, lz = st * np.cos(ph), st * np.sin(ph), ct
+    a, d = np.radians(ra_c), np.radians(dec_c)
+    sa, ca, sd, cd = np.sin(a), np.cos(a), np.sin(d), np.cos(d)
+    x = -sa * lx - sd * ca * ly + cd * ca * lz
+    y = ca * lx - sd * sa * ly + cd * sa * lz
+    z = cd * ly + sd * lz
+    ra = _wrap_ra(np.degrees(np.arctan2(y, x)))
+    dec = np.degrees(np.arcsin(np.clip(z, -1.0, 1.0)))
+    return ra, np.clip(dec, -90.0, 90.0)
+
+
+def batch_job_specs(seed=4, n_jobs=10_000, n_lo=1e2, n_hi=1e6, cap_lo=0.1, cap_hi=2.0,
+                    r_lo_arcsec=0.5, r_hi_arcsec=10.0):
+    """Parameters of a batch of independent joins: N_A, N_B ~ logU[n_lo, n_hi],
+    cap centre uniform on the sphere, cap radius U[cap_lo, cap_hi] deg, radius
+    ~ logU[r_lo, r_hi] arcsec, zone height = radius.  Returns a structured array."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    spec = np.empty(n_jobs, dtype=[("n_a", "<i8"), ("n_b", "<i8"), ("ra_c", "<f8"),
+                                   ("dec_c", "<f8"), ("cap_deg", "<f8"), ("r_deg", "<f8")])
+    lg = np.log10
+    spec["n_a"] = np.round(10.0 ** rng.uniform(lg(n_lo), lg(n_hi), n_jobs)).astype(np.int64)
+    spec["n_b"] = np.round(10.0 ** rng.uniform(lg(n_lo), lg(n_hi), n_jobs)).astype(np.int64)
+    ra, dec = uniform_sphere(rng, n_jobs)
+    spec["ra_c"], spec["dec_c"] = ra, dec
+    spec["cap_deg"] = rng.uniform(cap_lo, cap_hi, n_jobs)
+    spec["r_deg"] = 10.0 ** rng.uniform(lg(r_lo_arcsec), lg(r_hi_arcsec), n_jobs) * ARCSEC
+    return spec
+
+
+def batch_job_catalogs(spec, seed=4):
+    """Host catalogs of a batch: per job, A uniform in its cap; B 50% copies of A
+    (sigma 0.3") + 50% fresh in the cap.  Ids are unique across the batch."""
+    rng = np.random.Generator(np.random.PCG64([seed, 1]))
+    out = []
+    id0 = 0
+    for s in spec:
+        na, nb = int(s["n_a"]), int(s["n_b"])
+        ra, dec = uniform_cap(rng, na, s["ra_c"], s["dec_c"], s["cap_deg"])
+        a = make_objs(id0 + np.arange(na, dtype=np.int64), ra, dec)
+        b = copies_plus_fresh(rng, a, nb, 0.5, 0.3,
+                              lambda r, n, s=s: uniform_cap(r, n, s["ra_c"], s["dec_c"],
+                                                            s["cap_deg"]),
+                              id_offset=B_ID_OFFSET + id0) if na else make_objs(
+                                  B_ID_OFFSET + id0 + np.arange(nb, dtype=np.int64),
+                                  *uniform_cap(rng, nb, s["ra_c"], s["dec_c"], s["cap_deg"]))
+        id0 += max(na, nb)
+        out.append((a, b, float(s["r_deg"])))
+    return out
+
+
 def read_idradec(path, count=None, offset=0):
     """Raw little-endian 24-byte records, as Segment::Load reads them
     (BoostXmatch/BoostXmatch/Segment.cpp:14-22); count = file bytes / 24."""
