@@ -153,11 +153,148 @@ def run_plane(args):
     print(json.dumps(out), flush=True)
 
 
+def run_batch(args):
+    """Batched form: n_jobs independent joins (cap-shaped segment pairs of logU sizes,
+    each at its own radius, zone height = radius) as one many-segment catalog per
+    side and one batched join, against a loop of one-job calls (catalog A, catalog
+    B, join) over the same device-resident records."""
+    n_jobs = args.jobs
+    spec = synth.batch_job_specs(seed=4, n_jobs=n_jobs)
+    na, nb = spec["n_a"], spec["n_b"]
+    rs = [float(r) for r in spec["r_deg"]]
+    ctx = xm.Context(0)
+    t0 = time.time()
+    dA = ctx.alloc_records(int(na.sum()))
+    dB = ctx.alloc_records(int(nb.sum()))
+    a_views, b_views = [], []
+    oa = ob = 0
+    for j in range(n_jobs):
+        cap = xm.shape(xm.SHAPE_CAP, float(spec["ra_c"][j]), float(spec["dec_c"][j]),
+                       float(spec["cap_deg"][j]))
+        va = dA.view(oa, int(na[j]))
+        vb = dB.view(ob, int(nb[j]))
+        ctx.synth_fresh(cap, int(na[j]), 4000 + args.seed, stream=2 * j, id0=oa, out=va)
+        ctx.synth_copies(va, 0.5, 0.3, cap, int(nb[j]), 4000 + args.seed, stream=2 * j + 1,
+                         id0=10**11 + ob, out=vb)
+        a_views.append(va)
+        b_views.append(vb)
+        oa += int(na[j])
+        ob += int(nb[j])
+    log("generated %d jobs, %d + %d sources on the device in %.1f s"
+        % (n_jobs, oa, ob, time.time() - t0))
+    seg_a = xm.SegmentTable(a_views, rs)
+    seg_b = xm.SegmentTable(b_views, rs)
+    jobs = xm.job_table([(j, j, rs[j]) for j in range(n_jobs)])
+
+    def batched():
+        A = ctx.catalog_segments(seg_a)
+        B = ctx.catalog_segments(seg_b)
+        res = ctx.match_batch(A, B, jobs)
+        return A, B, res
+
+    ctx.set_timing(True)
+    for _ in range(args.warmup):
+        A, B, res = batched()
+        res.close(); A.close(); B.close()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ms = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        A, B, res = batched()
+        ms.append(1e3 * (time.perf_counter() - t))
+        if i < args.steps - 1:
+            res.close(); A.close(); B.close()
+    kernels = {}
+    for k in KERNELS + ["seg_range", "job_prep"]:
+        tot, cnt = ctx.kernel_time(k)
+        if cnt:
+            kernels[k] = {"total_ms": tot / args.steps, "launches": cnt // args.steps}
+    ctx.set_timing(False)
+    t_batch = float(np.median(ms))
+    stats = res.stats.as_dict()
+    _, counts_batch = res.jobs()
+    log("batched: %.1f ms per step, %d matches" % (t_batch, stats["n_matches"]))
+    # work counts of every job, from the sorted segment tables (host)
+    t0 = time.time()
+    w = {"w_ref": 0, "w_cand": 0, "w_tile": 0}
+    if not args.no_work_counts:
+        for j in range(n_jobs):
+            wj = ctx.work_counts_job(A, B, (j, j, rs[j]))
+            for k in w:
+                w[k] += wj[k]
+        log("work counts in %.1f s" % (time.time() - t0))
+    res.close(); A.close(); B.close()
+    ctx.release_memory()
+
+    def loop():
+        counts = np.zeros(n_jobs, np.uint64)
+        for j in range(n_jobs):
+            ca = ctx.catalog(a_views[j], rs[j])
+            cb = ctx.catalog(b_views[j], rs[j])
+            r = ctx.match(ca, cb, rs[j])
+            counts[j] = r.count
+            r.close(); cb.close(); ca.close()
+        return counts
+
+    loop()
+    ctx.synchronize()
+    t = time.perf_counter()
+    counts_loop = loop()
+    t_loop = 1e3 * (time.perf_counter() - t)
+    log("loop of one-job calls: %.1f ms" % t_loop)
+    # a sample of jobs compared pair by pair (batch of the sample vs one-job joins)
+    rng = np.random.default_rng(0)
+    sample = sorted(set(rng.choice(n_jobs, size=min(24, n_jobs), replace=False).tolist()))
+    sa = ctx.catalog_segments([a_views[j] for j in sample], [rs[j] for j in sample])
+    sb = ctx.catalog_segments([b_views[j] for j in sample], [rs[j] for j in sample])
+    sres = ctx.match_batch(sa, sb, [(i, i, rs[j]) for i, j in enumerate(sample)])
+    per_job = sres.by_job()
+    sample_ok = True
+    for i, j in enumerate(sample):
+        ca = ctx.catalog(a_views[j], rs[j])
+        cb = ctx.catalog(b_views[j], rs[j])
+        r1 = ctx.match(ca, cb, rs[j])
+        x, y, d = r1.to_host()
+        g = per_job[i]
+        sample_ok &= bool(np.array_equal(x, g[0]) and np.array_equal(y, g[1]) and
+                          (len(d) == 0 or np.abs(d - g[2]).max() <= 1e-6))
+        r1.close(); cb.close(); ca.close()
+    sres.close(); sa.close(); sb.close()
+    n_src = int(na.sum() + nb.sum())
+    out = {
+        "metric": "pair-tests/s", "unit": "pair-tests/s", "n_gpus": 1, "data": "synthetic",
+        "config": {"workload": "batched joins: %d cap-shaped segment pairs, N_A, N_B ~ logU[1e2, "
+                               "1e6], cap radius U[0.1, 2] deg, r ~ logU[0.5, 10] arcsec, h = r"
+                               % n_jobs,
+                   "n_jobs": n_jobs, "sources": n_src, "b": "50% copies (0.3 arcsec) + 50% fresh",
+                   "precision": "fp64_plain"},
+        "steps": args.steps, "warmup": args.warmup,
+        "batched_ms": t_batch, "loop_ms": t_loop, "batched_over_loop": t_loop / t_batch,
+        "value": (w["w_cand"] / (t_batch / 1e3)) if w["w_cand"] else None,
+        "loop_pair_tests_per_s": (w["w_cand"] / (t_loop / 1e3)) if w["w_cand"] else None,
+        "matches": stats["n_matches"], "matches_per_s": stats["n_matches"] / (t_batch / 1e3),
+        "sources_per_s": n_src / (t_batch / 1e3), "n_retries": stats["n_retries"],
+        "device_join_ms": stats["join_ms"],
+        "per_job_counts_equal": bool(np.array_equal(counts_batch, counts_loop)),
+        "sample_jobs_identical": sample_ok, "sample_jobs": len(sample),
+        "kernels": kernels,
+    }
+    out.update(w)
+    dA.free(); dB.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="full_sky", choices=["full_sky", "plane"],
+    ap.add_argument("--config", default="full_sky", choices=["full_sky", "plane", "batch"],
                     help="full_sky: the 50M x 50M FP64 join (default); plane: the 200M x 200M "
-                         "precision-form comparison")
+                         "precision-form comparison; batch: many independent joins, batched "
+                         "against a loop of one-job calls")
+    ap.add_argument("--jobs", type=int, default=10_000, help="jobs of the batch config")
+    ap.add_argument("--no-work-counts", action="store_true",
+                    help="batch config: skip the host work counts")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -185,6 +322,9 @@ def main():
         sys.exit(2)
     if args.config == "plane":
         run_plane(args)
+        return
+    if args.config == "batch":
+        run_batch(args)
         return
 
     t0 = time.time()

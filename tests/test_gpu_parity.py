@@ -103,6 +103,15 @@ def test_overflow_retry_never_truncates(gpu_ctx):
     assert_same(got, ref)
 
 
+def test_spill_pool_takes_region_overflow(gpu_ctx):
+    # ~70k matches over 1024 regions of ~73 slots each: many regions overflow into
+    # the shared spill pool, which holds the rest, so no re-run is needed
+    A, B, r = synth.config_small_patch(seed=1)
+    got, stats = gpu_join(gpu_ctx, A, B, r, capacity_hint=100_000)
+    assert stats["n_retries"] == 0
+    assert_same(got, O.match(A, B, r)[0])
+
+
 @pytest.mark.parametrize("where", ["seam", "north_pole", "south_pole", "dense_cluster", "mid"])
 def test_edge_geometry_vs_allpairs(gpu_ctx, where):
     rng = np.random.default_rng(abs(hash(where)) % 2**32)

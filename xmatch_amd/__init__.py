@@ -71,6 +71,14 @@ class Job(ctypes.Structure):
                 ("radius_deg", ctypes.c_double)]
 
 
+def job_table(jobs):
+    """A prepared xm_job array for Context.match_batch (at least one job)."""
+    jobs = list(jobs)
+    if not jobs:
+        raise ValueError("no jobs")
+    return _job_array(jobs)
+
+
 def cross_product_jobs(n_seg_a, n_seg_b, radius_deg):
     """Every A segment against every B segment, A-major, as the reference's
     JobManager builds its queue (JobManager.cpp:5-13)."""
@@ -78,6 +86,8 @@ def cross_product_jobs(n_seg_a, n_seg_b, radius_deg):
 
 
 def _job_array(jobs):
+    if isinstance(jobs, ctypes.Array):
+        return jobs
     arr = (Job * max(len(jobs), 1))()
     for i, (a, b, r) in enumerate(jobs):
         arr[i].seg_a = int(a)
@@ -253,17 +263,18 @@ class Context:
         self._check(lib().xm_device_alloc(self._h, max(nbytes, 1), ctypes.byref(d)))
         return DeviceBuffer(self, d, nbytes, OBJ_DTYPE, (int(n),))
 
-    def synth_fresh(self, shp, n, seed, stream=0, id0=0):
-        """n device records of shape shp (Philox-4x32-10 draws keyed by seed)."""
-        buf = self.alloc_records(n)
+    def synth_fresh(self, shp, n, seed, stream=0, id0=0, out=None):
+        """n device records of shape shp (Philox-4x32-10 draws keyed by seed), into
+        out (a record DeviceBuffer or view of n rows) or a new buffer."""
+        buf = self.alloc_records(n) if out is None else out
         self._check(lib().xm_synth_fresh(self._h, ctypes.byref(shp), int(n), seed, stream,
                                          int(id0), buf.ptr))
         return buf
 
     def synth_copies(self, a, frac_copy, sigma_arcsec, fresh, n, seed, stream=1,
-                     id0=B_ID_OFFSET):
+                     id0=B_ID_OFFSET, out=None):
         """n device records: perturbed copies of records of DeviceBuffer a, then fresh."""
-        buf = self.alloc_records(n)
+        buf = self.alloc_records(n) if out is None else out
         na = a.nbytes // OBJ_DTYPE.itemsize
         self._check(lib().xm_synth_copies(self._h, a.ptr, na, frac_copy, sigma_arcsec,
                                           ctypes.byref(fresh), int(n), seed, stream, int(id0),
@@ -280,10 +291,12 @@ class Context:
         """Zone-sorted device catalog from host records (dtype OBJ_DTYPE) or a DeviceBuffer."""
         return Catalog(self, records, zone_height_deg)
 
-    def catalog_segments(self, segments, zone_height_deg):
+    def catalog_segments(self, segments, zone_height_deg=None):
         """One catalog of many segments, prepared by one sort.  segments: host record
-        arrays (dtype OBJ_DTYPE) or DeviceBuffers (not mixed); zone_height_deg: one
-        value or one per segment."""
+        arrays (dtype OBJ_DTYPE) or DeviceBuffers (not mixed), or a SegmentTable;
+        zone_height_deg: one value or one per segment."""
+        if isinstance(segments, SegmentTable):
+            return Catalog(self, segments, None)
         return Catalog(self, list(segments), zone_height_deg)
 
     def match(self, a, b, radius_deg, precision=FP64_PLAIN, capacity_hint=0):
@@ -297,11 +310,12 @@ class Context:
         """One launch over jobs [(seg_a, seg_b, radius_deg), ...]; the result carries
         the job of every pair (Result.jobs)."""
         arr = _job_array(jobs)
+        n_jobs = getattr(jobs, "n_jobs", len(jobs))
         r = ctypes.c_void_p()
         stats = MatchStats()
-        self._check(lib().xm_match_batch(self._h, a.handle, b.handle, arr, len(jobs), precision,
+        self._check(lib().xm_match_batch(self._h, a.handle, b.handle, arr, n_jobs, precision,
                                          capacity_hint, ctypes.byref(r), ctypes.byref(stats)))
-        return Result(self, r, stats, n_jobs=len(jobs))
+        return Result(self, r, stats, n_jobs=n_jobs)
 
     def work_counts(self, a, b, radius_deg):
         w = [ctypes.c_uint64() for _ in range(3)]
@@ -318,44 +332,64 @@ class Context:
 
 
 class DeviceBuffer:
-    def __init__(self, ctx, ptr, nbytes, dtype, shape):
+    def __init__(self, ctx, ptr, nbytes, dtype, shape, owner=True):
         self.ctx, self.ptr, self.nbytes, self.dtype, self.shape = ctx, ptr, nbytes, dtype, shape
+        self.owner = owner
+
+    def view(self, first, count):
+        """Rows [first, first + count) of a 1-D buffer, without ownership."""
+        item = np.dtype(self.dtype).itemsize
+        if first < 0 or count < 0 or (first + count) * item > self.nbytes:
+            raise IndexError("view outside the buffer")
+        return DeviceBuffer(self.ctx, ctypes.c_void_p((self.ptr.value or 0) + first * item),
+                            count * item, self.dtype, (int(count),), owner=False)
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.owner:
             self.ctx._check(lib().xm_device_free(self.ctx.handle, self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+
+class SegmentTable:
+    """A prepared xm_segment array (e.g. built once, outside a timed loop)."""
+
+    def __init__(self, segments, zone_height_deg):
+        self.keep = []
+        n = len(segments)
+        hs = (list(zone_height_deg) if np.ndim(zone_height_deg) else
+              [float(zone_height_deg)] * n)
+        self.segs = (Segment * max(n, 1))()
+        self.n = n
+        self.on_device = n > 0 and isinstance(segments[0], DeviceBuffer)
+        for i, r in enumerate(segments):
+            if isinstance(r, DeviceBuffer) != self.on_device:
+                raise TypeError("segments must be all host arrays or all DeviceBuffers")
+            if self.on_device:
+                self.segs[i].records = r.ptr
+                self.segs[i].n = r.nbytes // OBJ_DTYPE.itemsize
+            else:
+                rec = np.ascontiguousarray(r)
+                if rec.dtype != OBJ_DTYPE:
+                    raise TypeError("records must have dtype %r" % (OBJ_DTYPE,))
+                self.keep.append(rec)
+                self.segs[i].records = rec.ctypes.data if rec.size else None
+                self.segs[i].n = rec.size
+            self.segs[i].zone_height_deg = hs[i]
 
 
 class Catalog:
     def __init__(self, ctx, records, zone_height_deg):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
-        if isinstance(records, list):
-            self._keep = []
-            n = len(records)
-            hs = (list(zone_height_deg) if np.ndim(zone_height_deg) else
-                  [float(zone_height_deg)] * n)
-            segs = (Segment * max(n, 1))()
-            on_device = n > 0 and isinstance(records[0], DeviceBuffer)
-            for i, r in enumerate(records):
-                if isinstance(r, DeviceBuffer) != on_device:
-                    raise TypeError("segments must be all host arrays or all DeviceBuffers")
-                if on_device:
-                    segs[i].records = r.ptr
-                    segs[i].n = r.nbytes // OBJ_DTYPE.itemsize
-                else:
-                    rec = np.ascontiguousarray(r)
-                    if rec.dtype != OBJ_DTYPE:
-                        raise TypeError("records must have dtype %r" % (OBJ_DTYPE,))
-                    self._keep.append(rec)
-                    segs[i].records = rec.ctypes.data if rec.size else None
-                    segs[i].n = rec.size
-                segs[i].zone_height_deg = hs[i]
-            fn = (lib().xm_catalog_create_segments_device if on_device
+        if isinstance(records, SegmentTable):
+            fn = (lib().xm_catalog_create_segments_device if records.on_device
                   else lib().xm_catalog_create_segments)
-            st = fn(ctx.handle, segs, n, ctypes.byref(self._h))
-            self._keep = None
+            st = fn(ctx.handle, records.segs, records.n, ctypes.byref(self._h))
+        elif isinstance(records, list):
+            t = SegmentTable(records, zone_height_deg)
+            fn = (lib().xm_catalog_create_segments_device if t.on_device
+                  else lib().xm_catalog_create_segments)
+            st = fn(ctx.handle, t.segs, t.n, ctypes.byref(self._h))
         elif isinstance(records, DeviceBuffer):
             n = records.nbytes // OBJ_DTYPE.itemsize
             st = lib().xm_catalog_create_device(ctx.handle, records.ptr, n, zone_height_deg,

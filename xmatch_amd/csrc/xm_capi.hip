@@ -792,6 +792,17 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
 
 namespace {
 
+// Above this much staging (24 bytes a slot) a join stages row indices (8 bytes a
+// slot) and the compaction gathers the ids and distances from the rows.
+constexpr uint64_t WIDE_STAGING_BYTES = uint64_t(8) << 30;
+
+void free_slots(xm_ctx* ctx, RegionOut& ro) {
+    dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
+    dfree(ctx, ro.ra); dfree(ctx, ro.rb); dfree(ctx, ro.job);
+    ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr;
+    ro.ra = nullptr; ro.rb = nullptr; ro.job = nullptr;
+}
+
 // The join of a batch of jobs, with overflow-and-retry.  batch = keep the job
 // column and the per-job counts in the result.
 xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, const xm_job* jobs,
@@ -805,8 +816,9 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
     res->ctx = ctx;
     RegionOut ro{};
     JobTable jt;
-    uint64_t* scratch = nullptr;  // prefix[n_regions] + tail[4]
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t* scratch = nullptr;
+    uint64_t* job_sums = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     xm_match_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
     xm_status st = guarded(ctx, [&] {
@@ -832,87 +844,123 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
                 cap += 2 * std::max<uint64_t>(A->seg[jobs[j].seg_a].n, b->seg[jobs[j].seg_b].n);
         }
         cap = std::max<uint64_t>(cap, 1024);
-        XM_HIP(hipEventCreate(&ev0));
-        XM_HIP(hipEventCreate(&ev1));
-        // Hits go to n_regions append regions (blocks spread over them round-robin),
-        // a one-block scan sizes them and k_compact packs them into the result; the
-        // only host round trip reads the total and the fullest region.
+        for (auto& e : ev) XM_HIP(hipEventCreate(&e));
+        // Hits go to n_regions append regions as (A row, B row[, job]); blocks spread
+        // over them round-robin.  The first run gives the regions 3/4 of the slots
+        // and keeps 1/4 as a shared spill pool, so uneven regions do not overflow
+        // while the total fits.  A one-block scan sizes the parts; one host round
+        // trip reads the total; k_compact then gathers ids and distances into a
+        // result of exactly the total.  If the pool overflowed, the re-run gives
+        // every region exactly its count from the first run (the block -> region
+        // assignment and each block's hits do not change), so it cannot overflow.
         ro.n_regions = std::max<uint32_t>(
             1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(std::max<uint64_t>(jt.max_waves, 1))));
-        ro.region_cap = std::max<uint64_t>(1, (cap + ro.n_regions - 1) / ro.n_regions);
+        const uint32_t R = ro.n_regions;
         ro.counters = static_cast<uint64_t*>(
-            dalloc(ctx, size_t(ro.n_regions) * REGION_STRIDE * sizeof(uint64_t)));
-        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(ro.n_regions) + 4) * sizeof(uint64_t)));
-        if (batch) ro.job_count = static_cast<unsigned long long*>(dalloc(ctx, n_jobs * 8));
+            dalloc(ctx, size_t(R + 1) * REGION_STRIDE * sizeof(uint64_t)));
+        // scratch: prefix[R + 2], tail[4], base[R + 1], cap[R], next_base[R + 1], next_cap[R]
+        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(R) * 5 + 8) * sizeof(uint64_t)));
         uint64_t* prefix = scratch;
-        uint64_t* tail = scratch + ro.n_regions;
+        uint64_t* tail = prefix + R + 2;
+        uint64_t* base = tail + 4;
+        uint64_t* capa = base + R + 1;
+        uint64_t* next_base = capa + R;
+        uint64_t* next_cap = next_base + R + 1;
+        if (batch) {
+            ro.n_jobs = uint32_t(n_jobs);
+            ro.job_count = static_cast<unsigned long long*>(dalloc(ctx, size_t(JOB_SPREAD) * n_jobs * 8));
+            job_sums = static_cast<uint64_t*>(dalloc(ctx, n_jobs * 8));
+        }
+        const uint64_t cap_each = std::max<uint64_t>(1, cap * 3 / 4 / R);
+        launch_region_init(ctx, R, cap_each, base, capa);
+        uint64_t slots = std::max<uint64_t>(cap, cap_each * R + 1);
+        ro.spill_cap = slots - cap_each * R;
         uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
+        uint64_t total = 0;
         for (int attempt = 0;; attempt++) {
-            const uint64_t slots = uint64_t(ro.n_regions) * ro.region_cap;
-            ro.id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
-            ro.id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
-            ro.d2 = static_cast<double*>(dalloc(ctx, slots * 8));
-            res->id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
-            res->id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
-            res->dist = static_cast<double*>(dalloc(ctx, slots * 8));
+            ro.base = base;
+            ro.cap = capa;
+            // wide slots unless the staging would pass WIDE_STAGING_BYTES
+            ro.wide = slots * 24 <= WIDE_STAGING_BYTES;
+            if (ro.wide) {
+                ro.id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+                ro.id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));
+                ro.d2 = static_cast<double*>(dalloc(ctx, slots * 8));
+            } else {
+                ro.ra = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
+                ro.rb = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
+            }
             if (batch) {
                 ro.job = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
-                res->job = static_cast<uint32_t*>(dalloc(ctx, slots * 4));
-                launch_fill_u64(ctx, reinterpret_cast<uint64_t*>(ro.job_count), n_jobs, 0);
+                launch_fill_u64(ctx, reinterpret_cast<uint64_t*>(ro.job_count),
+                                size_t(JOB_SPREAD) * n_jobs, 0);
             }
-            res->cap = slots;
-            launch_fill_u64(ctx, ro.counters, size_t(ro.n_regions) * REGION_STRIDE, 0);
-            XM_HIP(hipEventRecord(ev0, ctx->stream));
+            launch_fill_u64(ctx, ro.counters, size_t(R + 1) * REGION_STRIDE, 0);
+            XM_HIP(hipEventRecord(ev[0], ctx->stream));
             launch_pair(ctx, A, b, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
-            launch_region_scan(ctx, ro, prefix, tail);
-            launch_compact(ctx, ro, prefix, res->id_a, res->id_b, res->dist, res->job);
-            XM_HIP(hipEventRecord(ev1, ctx->stream));
+            launch_region_scan(ctx, ro, prefix, tail, next_base, next_cap);
+            XM_HIP(hipEventRecord(ev[1], ctx->stream));
             XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
             XM_HIP(hipMemcpyAsync(pin + 4, jt.prefix + jt.n_jobs, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
-            dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2); dfree(ctx, ro.job);
-            ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr; ro.job = nullptr;
             sync(ctx);
             float ms = 0.f;
-            XM_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            XM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
             st_local.join_ms += ms;
-            const uint64_t total = pin[0], most = pin[1];
+            total = pin[0];
+            const uint64_t spill = pin[1];
             st_local.n_chunks = reinterpret_cast<uint32_t*>(pin + 4)[0];
-            if (most <= ro.region_cap) {
-                res->n = total;
+            if (spill <= ro.spill_cap) {
                 st_local.band_hits = pin[2];
                 double err = 0.0;
                 std::memcpy(&err, &pin[3], sizeof(double));
                 st_local.max_dist_err_arcsec = err * RAD2DEG * 3600.0;
                 break;
             }
-            // overflow: every counter holds its region's exact total; re-run with
-            // regions that fit the fullest one
+            // the pool overflowed: re-run with every region sized to its count
             st_local.n_retries++;
-            dfree(ctx, res->id_a); dfree(ctx, res->id_b); dfree(ctx, res->dist); dfree(ctx, res->job);
-            res->id_a = nullptr; res->id_b = nullptr; res->dist = nullptr; res->job = nullptr;
-            ro.region_cap = most;
-            if (attempt > 2) throw std::runtime_error("result size did not converge");
+            if (attempt > 0) throw std::runtime_error("result size did not converge");
+            free_slots(ctx, ro);
+            std::swap(base, next_base);
+            std::swap(capa, next_cap);
+            slots = std::max<uint64_t>(total, 1);
+            ro.spill_cap = 0;
         }
+        const uint64_t slots_out = std::max<uint64_t>(total, 1);
+        res->id_a = static_cast<int64_t*>(dalloc(ctx, slots_out * 8));
+        res->id_b = static_cast<int64_t*>(dalloc(ctx, slots_out * 8));
+        res->dist = static_cast<double*>(dalloc(ctx, slots_out * 8));
+        if (batch) res->job = static_cast<uint32_t*>(dalloc(ctx, slots_out * 4));
+        res->cap = slots_out;
+        res->n = total;
+        XM_HIP(hipEventRecord(ev[2], ctx->stream));
+        launch_compact(ctx, ro, prefix, total, A, b, res->id_a, res->id_b, res->dist, res->job);
+        XM_HIP(hipEventRecord(ev[3], ctx->stream));
         if (batch) {
+            launch_job_sum(ctx, ro.job_count, uint32_t(n_jobs), job_sums);
             res->job_counts.resize(n_jobs);
-            XM_HIP(hipMemcpyAsync(res->job_counts.data(), ro.job_count, n_jobs * 8,
+            XM_HIP(hipMemcpyAsync(res->job_counts.data(), job_sums, n_jobs * 8,
                                   hipMemcpyDeviceToHost, ctx->stream));
-            sync(ctx);
         }
+        free_slots(ctx, ro);
+        sync(ctx);
+        float ms = 0.f;
+        XM_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        st_local.join_ms += ms;
         st_local.n_matches = res->n;
     });
     (void)guarded(ctx, [&] {
         sync(ctx);
         dfree(ctx, ro.counters);
         dfree(ctx, scratch);
-        dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2); dfree(ctx, ro.job);
+        free_slots(ctx, ro);
         dfree(ctx, reinterpret_cast<void*>(ro.job_count));
+        dfree(ctx, job_sums);
         dfree(ctx, jt.jobs);
         dfree(ctx, jt.prefix);
-        if (ev0) XM_HIP(hipEventDestroy(ev0));
-        if (ev1) XM_HIP(hipEventDestroy(ev1));
+        for (auto e : ev)
+            if (e) XM_HIP(hipEventDestroy(e));
     });
     if (stats) *stats = st_local;
     if (st != XM_OK) {

@@ -209,22 +209,35 @@ struct PairParams {
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
 };
-// Hits are appended to n_regions regions of region_cap slots; block b appends to
-// region b % n_regions through counters[region * REGION_STRIDE] (one 128-byte line
-// per counter, so the waves of a join never queue on one address).  Counters count
-// every hit; slots past region_cap are dropped and the caller re-runs.
+// Output of a join.  Hits are appended to n_regions regions; block b appends to
+// region b % n_regions through counters[r * REGION_STRIDE] (one 128-byte line per
+// counter, so the waves of a join never queue on one address).  Region r owns
+// slots [base[r], base[r] + cap[r]); a flush that runs past its region's end puts
+// the rest in the spill pool [base[n_regions], + spill_cap) through the spill
+// counter counters[n_regions * REGION_STRIDE].  Counters count every hit, so when
+// the spill pool overflows the caller re-runs with each region sized to its count.
 constexpr uint32_t MAX_REGIONS = 1024;
 constexpr uint32_t REGION_STRIDE = 16;
+constexpr uint32_t JOB_SPREAD = 16;  // partial hit counters per job, on separate lines
 struct RegionOut {
-    uint64_t* counters;
+    uint64_t* counters;      // [(n_regions + 1) * REGION_STRIDE]
     uint32_t n_regions;
-    uint64_t region_cap;
-    int64_t* id_a;   // [n_regions * region_cap]
+    const uint64_t* base;    // [n_regions + 1]: first slot of each region, then of the pool
+    const uint64_t* cap;     // [n_regions]: slots of each region
+    uint64_t spill_cap;
+    // wide slots hold (idA, idB, chord^2), written while the rows are in cache;
+    // narrow slots hold (A row, B row), 8 bytes instead of 24, for very large outputs
+    bool wide;
+    int64_t* id_a;           // wide: [slots]
     int64_t* id_b;
-    double* d2;      // chord^2 of each hit
-    uint32_t* job;   // batch joins: job of each hit (else null)
-    unsigned long long* job_count;  // batch joins: hits per job (else null)
+    double* d2;
+    uint32_t* ra;            // narrow: [slots]: A row of each hit
+    uint32_t* rb;            // B row of each hit
+    uint32_t* job;           // batch joins: job of each hit (else null)
+    unsigned long long* job_count;  // batch joins: [JOB_SPREAD * n_jobs] partial counts
+    uint32_t n_jobs;
 };
+
 // One join of a batch: A segment x B segment at one radius.  The host fills
 // everything but a_chunk_base; launch_job_prep fills that and the wave counts.
 struct JobDesc {
@@ -248,14 +261,23 @@ void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Jo
                  const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
                  const RegionOut& ro);
 void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
-// prefix[r] = hits of the regions before r (capped at region_cap each);
-// tail[0] = total hits, tail[1] = largest region count, tail[2] = FP32 band hits,
-// tail[3] = largest |chord_32 - chord_64| of a kept pair (double bits).
-void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail);
-// Packs the regions into contiguous arrays (region r's hits at prefix[r]) and
-// converts chord^2 to the FP64 distance in arcsec; job (may be null) gets the
-// job column of batch joins.
-void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
-                    int64_t* id_b, double* dist, uint32_t* job);
+// Regions of cap_each slots back to back, the spill pool after them.
+void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t* base,
+                        uint64_t* cap);
+// prefix[r] (r <= n_regions) = kept hits of the regions before r, so the pool's kept
+// hits start at prefix[n_regions]; tail[0] = all hits, tail[1] = spill demand,
+// tail[2] = FP32 band hits, tail[3] = largest |chord_32 - chord_64| (double bits);
+// next_base / next_cap = regions sized to their counts, for a re-run.
+void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail,
+                        uint64_t* next_base, uint64_t* next_cap);
+// job_counts[j] = sum of the JOB_SPREAD partial counts of job j.
+void launch_job_sum(xm_ctx* ctx, const unsigned long long* partial, uint32_t n_jobs,
+                    uint64_t* job_counts);
+// Packs the regions' total hits into contiguous arrays (region r's hits at
+// prefix[r]): ids from the rows, FP64 distance in arcsec; job (may be null) gets
+// the job column of batch joins.
+void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, uint64_t total,
+                    const xm_catalog* A, const xm_catalog* B, int64_t* id_a, int64_t* id_b,
+                    double* dist, uint32_t* job);
 
 }  // namespace xm

@@ -118,7 +118,7 @@ template <> struct Prec<true> {
 struct Stage {
     uint32_t* ra;
     uint32_t* rb;
-    double* d2;
+    double* d2;  // chord^2 of the test (FP32 form: the filter's, for the band statistics)
     uint32_t n;  // wave-uniform fill
 };
 
@@ -126,82 +126,118 @@ struct Out {
     const Row64* __restrict__ arow;
     const Row64* __restrict__ brow;
     unsigned long long* __restrict__ counter;  // this block's region counter line
-    uint64_t region_cap;
-    int64_t* __restrict__ id_a;                // this block's region
+    unsigned long long* __restrict__ spill;    // the spill pool's counter
+    uint64_t base, cap;                        // this block's region slots
+    uint64_t spill_base, spill_cap;
+    bool wide;
+    int64_t* __restrict__ id_a;                // wide slot arrays
     int64_t* __restrict__ id_b;
     double* __restrict__ d2;
+    uint32_t* __restrict__ ra;                 // narrow slot arrays
+    uint32_t* __restrict__ rb;
+    uint32_t* __restrict__ job;                // batch joins: job column
+    unsigned long long* __restrict__ job_count;  // batch joins: a partial counter of this job
     double thr;                                // FP64 chord threshold
     double t_lo;                               // FP32 form: below it the filter is sure
-    uint32_t* __restrict__ job;                // batch joins: this region's job column
-    unsigned long long* __restrict__ job_count;  // batch joins: this job's hit counter
     uint32_t jid;
 };
 
 __device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro,
                                const PairParams& pp, uint32_t jid) {
     const uint32_t r = blockIdx.x % ro.n_regions;
-    const uint64_t base = uint64_t(r) * ro.region_cap;
-    return Out{arow, brow, (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE),
-               ro.region_cap, ro.id_a + base, ro.id_b + base, ro.d2 + base, pp.chord2,
-               double(pp.t_lo32), ro.job ? ro.job + base : nullptr,
-               ro.job_count ? ro.job_count + jid : nullptr, jid};
+    Out o;
+    o.arow = arow;
+    o.brow = brow;
+    o.counter = (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE);
+    o.spill = (unsigned long long*)(ro.counters + size_t(ro.n_regions) * REGION_STRIDE);
+    o.base = ro.base[r];
+    o.cap = ro.cap[r];
+    o.spill_base = ro.base[ro.n_regions];
+    o.spill_cap = ro.spill_cap;
+    o.wide = ro.wide;
+    o.id_a = ro.id_a;
+    o.id_b = ro.id_b;
+    o.d2 = ro.d2;
+    o.ra = ro.ra;
+    o.rb = ro.rb;
+    o.job = ro.job;
+    o.job_count = ro.job_count
+                      ? ro.job_count + size_t(blockIdx.x % JOB_SPREAD) * ro.n_jobs + jid
+                      : nullptr;
+    o.thr = pp.chord2;
+    o.t_lo = double(pp.t_lo32);
+    o.jid = jid;
+    return o;
 }
 
-// Write the wave's staged hits to its region: one atomic per call.  In the FP32
-// form each staged pair is re-evaluated in FP64 here and dropped if it fails; the
-// band-hit count and the largest |chord_32 - chord_64| go to the region's line.
+// Write the wave's staged hits to its region: one atomic per call.  Wide slots get
+// (idA, idB, chord^2) -- the rows were just read by the test, so the id loads hit
+// in cache -- narrow slots get (A row, B row).  In the FP64 form every staged pair
+// is a match.  In the FP32 form each staged pair is re-evaluated in FP64 here and
+// dropped if it fails; the band-hit count and the largest |chord_32 - chord_64| go
+// to the region's line.
 template <bool F32>
 __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
     wave_lds_sync();
-    bool keep[STAGE / WAVE];
-    double dd[STAGE / WAVE];
-    int64_t ia[STAGE / WAVE], ib[STAGE / WAVE];
     uint64_t mk[STAGE / WAVE];
     uint32_t total = 0, band = 0;
     double err = 0.0;
 #pragma unroll
     for (int k = 0; k < STAGE / WAVE; k++) {
         const uint32_t e = lane + k * WAVE;
-        const bool valid = e < st.n;
-        keep[k] = valid;
-        dd[k] = 0.0;
-        ia[k] = 0;
-        ib[k] = 0;
-        if (valid) {
+        bool keep = e < st.n;
+        if (F32 && keep) {
+            // the FP64 chord replaces the filter's in the stage (each lane owns its e)
             const Row64 a = o.arow[st.ra[e]];
             const Row64 b = o.brow[st.rb[e]];
-            ia[k] = a.id;
-            ib[k] = b.id;
-            dd[k] = st.d2[e];
-            if (F32) {
-                const double d32 = dd[k];
-                dd[k] = chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z);
-                keep[k] = dd[k] < o.thr;
-                band += d32 >= o.t_lo ? 1u : 0u;
-                if (keep[k]) {
-                    const double e1 = __builtin_fabs(__builtin_sqrt(d32) - __builtin_sqrt(dd[k]));
-                    err = e1 > err ? e1 : err;
-                }
+            const double d32 = st.d2[e];
+            const double d64 = chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z);
+            st.d2[e] = d64;
+            keep = d64 < o.thr;
+            band += d32 >= o.t_lo ? 1u : 0u;
+            if (keep) {
+                const double e1 = __builtin_fabs(__builtin_sqrt(d32) - __builtin_sqrt(d64));
+                err = e1 > err ? e1 : err;
             }
         }
-        mk[k] = __ballot(keep[k]);
+        mk[k] = __ballot(keep);
         total += uint32_t(__popcll(mk[k]));
     }
-    unsigned long long base = 0;
+    // positions [pos, pos + total) of the region's hit stream; those at or past the
+    // region's capacity go to the spill pool, reserved with a second atomic
+    unsigned long long pos = 0, sp = 0;
     if (lane == 0 && total) {
-        base = atomicAdd(o.counter, (unsigned long long)total);
+        pos = atomicAdd(o.counter, (unsigned long long)total);
         if (o.job_count) atomicAdd(o.job_count, (unsigned long long)total);
+        const uint64_t first_over = pos > o.cap ? pos : o.cap;
+        if (pos + total > o.cap)
+            sp = atomicAdd(o.spill, (unsigned long long)(pos + total - first_over));
     }
-    base = __shfl(base, 0, WAVE);
+    pos = __shfl(pos, 0, WAVE);
+    sp = __shfl(sp, 0, WAVE);
+    const uint64_t first_over = pos > o.cap ? uint64_t(pos) : o.cap;
     uint32_t before = 0;
 #pragma unroll
     for (int k = 0; k < STAGE / WAVE; k++) {
-        if (keep[k]) {
-            const uint64_t slot = uint64_t(base) + before + mbcnt64(mk[k]);
-            if (slot < o.region_cap) {
-                o.id_a[slot] = ia[k];
-                o.id_b[slot] = ib[k];
-                o.d2[slot] = dd[k];
+        if ((mk[k] >> lane) & 1u) {
+            const uint32_t e = lane + k * WAVE;
+            const uint64_t p = uint64_t(pos) + before + mbcnt64(mk[k]);
+            uint64_t slot = ~uint64_t(0);
+            if (p < o.cap) {
+                slot = o.base + p;
+            } else {
+                const uint64_t q = uint64_t(sp) + (p - first_over);
+                if (q < o.spill_cap) slot = o.spill_base + q;
+            }
+            if (slot != ~uint64_t(0)) {
+                if (o.wide) {
+                    o.id_a[slot] = o.arow[st.ra[e]].id;
+                    o.id_b[slot] = o.brow[st.rb[e]].id;
+                    o.d2[slot] = st.d2[e];
+                } else {
+                    o.ra[slot] = st.ra[e];
+                    o.rb[slot] = st.rb[e];
+                }
                 if (o.job) o.job[slot] = o.jid;
             }
         }
@@ -246,30 +282,56 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// Region r's hits [r*region_cap, +count_r) -> [prefix[r], +count_r), with chord^2
-// turned into 2 asin(sqrt(d2)/2) in arcsec (FP64).  One block per region.
-__global__ __launch_bounds__(256) void k_compact(const uint64_t* __restrict__ counters,
-                                                 uint64_t region_cap,
-                                                 const int64_t* __restrict__ ta,
-                                                 const int64_t* __restrict__ tb,
-                                                 const double* __restrict__ td,
-                                                 const uint32_t* __restrict__ tj,
+// Output row i of the packed result comes from part r (a region, or the spill
+// pool as part n_regions; prefix[r] <= i < prefix[r+1]) at slot base[r] + i -
+// prefix[r].  Wide slots (wa != null) hold the ids and chord^2; for narrow slots
+// the pair's rows give the ids and the FP64 chord -- the same chord2<double> of the
+// same two unit vectors the pair kernel tested, so the bits agree.  chord^2 turns
+// into 2 asin(sqrt(d2)/2) in arcsec.  The part prefix is searched in LDS.
+__global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
                                                  const uint64_t* __restrict__ prefix,
+                                                 const uint64_t* __restrict__ base,
+                                                 uint64_t total,
+                                                 const int64_t* __restrict__ wa,
+                                                 const int64_t* __restrict__ wb,
+                                                 const double* __restrict__ wd,
+                                                 const uint32_t* __restrict__ ta,
+                                                 const uint32_t* __restrict__ tb,
+                                                 const uint32_t* __restrict__ tj,
+                                                 const Row64* __restrict__ arow,
+                                                 const Row64* __restrict__ brow,
                                                  int64_t* __restrict__ id_a,
                                                  int64_t* __restrict__ id_b,
                                                  double* __restrict__ dist,
                                                  uint32_t* __restrict__ job) {
-    const uint32_t r = blockIdx.x;
-    uint64_t n = counters[size_t(r) * REGION_STRIDE];
-    n = n < region_cap ? n : region_cap;
-    const uint64_t src = uint64_t(r) * region_cap, dst = prefix[r];
-    for (uint64_t i = threadIdx.x; i < n; i += 256) {
-        id_a[dst + i] = ta[src + i];
-        id_b[dst + i] = tb[src + i];
-        dist[dst + i] = dist_arcsec(td[src + i]);
+    __shared__ uint64_t s_pre[MAX_REGIONS + 1];
+    __shared__ uint64_t s_base[MAX_REGIONS + 1];
+    for (uint32_t r = threadIdx.x; r < n_parts; r += 256) {
+        s_pre[r] = prefix[r];
+        s_base[r] = base[r];
     }
-    if (job)
-        for (uint64_t i = threadIdx.x; i < n; i += 256) job[dst + i] = tj[src + i];
+    __syncthreads();
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= total) return;
+    uint32_t lo = 0, hi = n_parts - 1;  // last part with prefix <= i
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (s_pre[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint64_t src = s_base[lo] + (i - s_pre[lo]);
+    if (wa) {
+        id_a[i] = wa[src];
+        id_b[i] = wb[src];
+        dist[i] = dist_arcsec(wd[src]);
+    } else {
+        const Row64 a = arow[ta[src]];
+        const Row64 b = brow[tb[src]];
+        id_a[i] = a.id;
+        id_b[i] = b.id;
+        dist[i] = dist_arcsec(chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z));
+    }
+    if (job) job[i] = tj[src];
 }
 
 // --------------------------------------------------------------- sparse ----
@@ -497,47 +559,82 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     if (st.n) flush_stage<F32>(st, lane, o);
 }
 
-// prefix[r] = sum of the capped counts of regions < r; tail[0] = total hits,
-// tail[1] = largest region count (uncapped), tail[2] = band hits, tail[3] = the
-// largest chord error (double bits).  One block.
+// One block.  kept_r = min(count_r, cap_r) for the regions and min(spill, spill_cap)
+// for the pool (part n_regions); prefix = exclusive scan of kept over the parts;
+// tail[0] = sum of count_r (every hit), tail[1] = spill demand, tail[2] = band hits,
+// tail[3] = the largest chord error (double bits); next_base = exclusive scan of
+// count_r (the pool after them, empty), next_cap = count_r.
 __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict__ counters,
-                                                      uint32_t n_regions, uint64_t region_cap,
+                                                      uint32_t n_regions,
+                                                      const uint64_t* __restrict__ cap,
+                                                      uint64_t spill_cap,
                                                       uint64_t* __restrict__ prefix,
-                                                      uint64_t* __restrict__ tail) {
-    __shared__ uint64_t s_sum[1024];
-    __shared__ uint64_t s_max[1024];
+                                                      uint64_t* __restrict__ tail,
+                                                      uint64_t* __restrict__ next_base,
+                                                      uint64_t* __restrict__ next_cap) {
+    __shared__ uint64_t s_kept[1024];
+    __shared__ uint64_t s_cnt[1024];
     __shared__ uint64_t s_band[1024];
     __shared__ uint64_t s_err[1024];
     const uint32_t t = threadIdx.x;
     const bool v = t < n_regions;
     const uint64_t c = v ? counters[size_t(t) * REGION_STRIDE] : 0;
-    s_sum[t] = c < region_cap ? c : region_cap;
-    s_max[t] = c;
+    const uint64_t kept = v ? (c < cap[t] ? c : cap[t]) : 0;
+    s_kept[t] = kept;
+    s_cnt[t] = c;
     s_band[t] = v ? counters[size_t(t) * REGION_STRIDE + 1] : 0;
     s_err[t] = v ? counters[size_t(t) * REGION_STRIDE + 2] : 0;
     __syncthreads();
-    // Hillis-Steele inclusive scan of the capped counts; the rest by tree reduction
+    // Hillis-Steele inclusive scans of kept and of count; tree reductions of the rest
     for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint64_t x = t >= d ? s_sum[t - d] : 0;
+        const uint64_t x = t >= d ? s_kept[t - d] : 0;
+        const uint64_t y = t >= d ? s_cnt[t - d] : 0;
         __syncthreads();
-        s_sum[t] += x;
+        s_kept[t] += x;
+        s_cnt[t] += y;
         __syncthreads();
     }
+    if (v) {
+        prefix[t] = s_kept[t] - kept;
+        next_base[t] = s_cnt[t] - c;
+        next_cap[t] = c;
+    }
+    __syncthreads();
     for (uint32_t d = 512; d >= 1; d >>= 1) {
         if (t < d) {
-            if (s_max[t + d] > s_max[t]) s_max[t] = s_max[t + d];
             if (s_err[t + d] > s_err[t]) s_err[t] = s_err[t + d];
             s_band[t] += s_band[t + d];
         }
         __syncthreads();
     }
-    if (v) prefix[t] = s_sum[t] - (c < region_cap ? c : region_cap);
     if (t == 0) {
-        tail[0] = s_sum[1023];
-        tail[1] = s_max[0];
+        const uint64_t spill = counters[size_t(n_regions) * REGION_STRIDE];
+        prefix[n_regions] = s_kept[1023];
+        prefix[n_regions + 1] = s_kept[1023] + (spill < spill_cap ? spill : spill_cap);
+        next_base[n_regions] = s_cnt[1023];
+        tail[0] = s_cnt[1023];
+        tail[1] = spill;
         tail[2] = s_band[0];
         tail[3] = s_err[0];
     }
+}
+
+__global__ __launch_bounds__(256) void k_region_init(uint32_t n_regions, uint64_t cap_each,
+                                                     uint64_t* __restrict__ base,
+                                                     uint64_t* __restrict__ cap) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r > n_regions) return;
+    base[r] = uint64_t(r) * cap_each;
+    if (r < n_regions) cap[r] = cap_each;
+}
+
+__global__ __launch_bounds__(256) void k_job_sum(const unsigned long long* __restrict__ partial,
+                                                 uint32_t n_jobs, uint64_t* __restrict__ out) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_jobs) return;
+    uint64_t s = 0;
+    for (uint32_t p = 0; p < JOB_SPREAD; p++) s += partial[size_t(p) * n_jobs + j];
+    out[j] = s;
 }
 
 __global__ __launch_bounds__(256) void k_job_prep(JobDesc* __restrict__ jobs, uint32_t n_jobs,
@@ -566,14 +663,36 @@ __global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, 
 
 }  // namespace
 
-void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, int64_t* id_a,
-                    int64_t* id_b, double* dist, uint32_t* job) {
+void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, uint64_t total,
+                    const xm_catalog* A, const xm_catalog* B, int64_t* id_a, int64_t* id_b,
+                    double* dist, uint32_t* job) {
+    if (total == 0) return;
     launch_timed(ctx, "compact", [&] {
-        hipLaunchKernelGGL(k_compact, dim3(ro.n_regions), dim3(256), 0, ctx->stream,
-                           (const uint64_t*)ro.counters, ro.region_cap, (const int64_t*)ro.id_a,
+        hipLaunchKernelGGL(k_compact, dim3(uint32_t((total + 255) / 256)), dim3(256), 0,
+                           ctx->stream, ro.n_regions + 1, prefix, ro.base, total,
+                           (const int64_t*)(ro.wide ? ro.id_a : nullptr),
                            (const int64_t*)ro.id_b, (const double*)ro.d2,
-                           (const uint32_t*)ro.job, prefix, id_a, id_b, dist,
+                           (const uint32_t*)ro.ra, (const uint32_t*)ro.rb,
+                           (const uint32_t*)ro.job, static_cast<const Row64*>(A->row),
+                           static_cast<const Row64*>(B->row), id_a, id_b, dist,
                            ro.job ? job : nullptr);
+    });
+}
+
+void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t* base,
+                        uint64_t* cap) {
+    launch_timed(ctx, "fill", [&] {
+        hipLaunchKernelGGL(k_region_init, dim3((n_regions + 1 + 255) / 256), dim3(256), 0,
+                           ctx->stream, n_regions, cap_each, base, cap);
+    });
+}
+
+void launch_job_sum(xm_ctx* ctx, const unsigned long long* partial, uint32_t n_jobs,
+                    uint64_t* job_counts) {
+    if (n_jobs == 0) return;
+    launch_timed(ctx, "job_sum", [&] {
+        hipLaunchKernelGGL(k_job_sum, dim3((n_jobs + 255) / 256), dim3(256), 0, ctx->stream,
+                           partial, n_jobs, job_counts);
     });
 }
 
@@ -588,10 +707,12 @@ void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t
     });
 }
 
-void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail) {
+void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail,
+                        uint64_t* next_base, uint64_t* next_cap) {
     launch_timed(ctx, "region_scan", [&] {
         hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1024), 0, ctx->stream,
-                           (const uint64_t*)ro.counters, ro.n_regions, ro.region_cap, prefix, tail);
+                           (const uint64_t*)ro.counters, ro.n_regions, ro.cap, ro.spill_cap,
+                           prefix, tail, next_base, next_cap);
     });
 }
 
