@@ -213,7 +213,7 @@ def run_batch(args):
     ctx.set_timing(False)
     t_batch = float(np.median(ms))
     stats = res.stats.as_dict()
-    _, counts_batch = res.jobs()
+    _, counts_batch = res.jobs(column=False)
     log("batched: %.1f ms per step, %d matches" % (t_batch, stats["n_matches"]))
     # work counts of every job, from the sorted segment tables (host)
     t0 = time.time()

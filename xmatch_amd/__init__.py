@@ -460,10 +460,10 @@ class Result:
         self.n_jobs = n_jobs
         ctx._children.add(self)
 
-    def jobs(self):
-        """Batch results: (job index of each pair, in to_host(sort=False) order,
-        pairs per job)."""
-        job = np.empty(self.count, np.uint32)
+    def jobs(self, column=True):
+        """Batch results: (job index of each pair, in to_host(sort=False) order, or
+        None without column; pairs per job)."""
+        job = np.empty(self.count, np.uint32) if column else None
         counts = np.empty(self.n_jobs or 0, np.uint64)
         self.ctx._check(lib().xm_result_jobs(self.ctx.handle, self._h, _ptr(job), _ptr(counts)))
         return job, counts
