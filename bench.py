@@ -286,12 +286,145 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def make_serving_queries(ctx, res_cat, res_host, n_queries, seed):
+    """Requests of the serving form: cap radius U[1, 30] arcmin around a random
+    resident source, N ~ logU[1, 1e5] sources, 70% of them 0.5" copies of resident
+    sources inside the cap, 30% fresh in the cap; radius r ~ U[1, 5] arcsec.  The
+    resident sources inside each cap come from one batched join of the cap centres
+    with the resident catalog."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_res = len(res_host)
+    centre = rng.integers(0, n_res, n_queries)
+    cap_deg = rng.uniform(1.0, 30.0, n_queries) / 60.0
+    nq = np.maximum(1, np.round(10.0 ** rng.uniform(0.0, 5.0, n_queries))).astype(np.int64)
+    r_deg = rng.uniform(1.0, 5.0, n_queries) / 3600.0
+    h = res_cat.info()["zone_height_deg"]
+    segs = [synth.make_objs([i], [res_host["ra"][c]], [res_host["dec"][c]])
+            for i, c in enumerate(centre)]
+    cc = ctx.catalog_segments(segs, h)
+    res = ctx.match_batch(cc, res_cat, [(i, 0, float(cap_deg[i])) for i in range(n_queries)],
+                          capacity_hint=1 << 26)
+    _, in_cap, _ = res.to_host(sort=False)
+    job, counts = res.jobs()
+    res.close(); cc.close()
+    o = np.argsort(job, kind="stable")
+    in_cap = in_cap[o]
+    edges = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    queries = []
+    id0 = 10**12
+    for i in range(n_queries):
+        n = int(nq[i])
+        k = int(round(0.7 * n))
+        members = in_cap[edges[i]:edges[i + 1]]
+        if len(members) == 0:
+            k = 0
+        ra, dec = synth.uniform_cap(rng, n, res_host["ra"][centre[i]], res_host["dec"][centre[i]],
+                                    cap_deg[i])
+        if k:
+            pick = members[rng.integers(0, len(members), k)]
+            ra[:k], dec[:k] = synth.perturb(rng, res_host["ra"][pick], res_host["dec"][pick], 0.5)
+        queries.append((synth.make_objs(id0 + np.arange(n), ra, dec), float(r_deg[i])))
+        id0 += n
+    return queries
+
+
+def run_serve(args):
+    """Serving form: a resident 5e8-source catalog (galactic-plane density, h = 1")
+    answers requests arriving as a Poisson process; latency = submit -> done,
+    measured by the server; p50 / p99 per arrival rate."""
+    import threading
+    n_res = args.n if args.n != 50_000_000 else 500_000_000
+    ctx = xm.Context(0)
+    plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
+    t0 = time.time()
+    d_res = ctx.synth_fresh(plane, n_res, 5000 + args.seed, stream=0, id0=0)
+    res_cat = ctx.catalog(d_res, 1.0 / 3600)
+    res_host = ctx.download(d_res)
+    d_res.free()
+    log("resident catalog: %d sources, built in %.1f s" % (n_res, time.time() - t0))
+    rates = [float(x) for x in args.rates.split(",")]
+    per_rate = []
+    rng = np.random.Generator(np.random.PCG64(args.seed + 77))
+    for rate in rates:
+        n_q = max(1, int(round(rate * args.duration)))
+        t0 = time.time()
+        queries = make_serving_queries(ctx, res_cat, res_host, n_q, seed=int(rate * 1000) + args.seed)
+        log("rate %.0f q/s: %d requests generated in %.1f s" % (rate, n_q, time.time() - t0))
+        gaps = rng.exponential(1.0 / rate, n_q)
+        with xm.Server(ctx, res_cat) as srv:
+            # warm the server's buffers with a few requests
+            for q, r in queries[:4]:
+                srv.result(srv.submit(q, r))
+            s0 = srv.stats()
+            tickets = [None] * n_q
+            lat = np.zeros(n_q)
+            nm = np.zeros(n_q, np.int64)
+            done = threading.Event()
+
+            def collect():
+                for i in range(n_q):
+                    while tickets[i] is None:
+                        time.sleep(0.0002)
+                    ia, _, _, l = srv.result(tickets[i])
+                    lat[i] = l
+                    nm[i] = len(ia)
+                done.set()
+
+            th = threading.Thread(target=collect)
+            th.start()
+            start = time.perf_counter()
+            t_next = start
+            for i, (q, r) in enumerate(queries):
+                t_next += gaps[i]
+                wait = t_next - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                tickets[i] = srv.submit(q, r)
+            done.wait()
+            th.join()
+            wall = time.perf_counter() - start
+            s1 = srv.stats()
+        sizes = np.array([len(q) for q, _ in queries])
+        small = sizes <= 10_000
+        batches = s1["batches"] - s0["batches"]
+        per_rate.append({
+            "rate_qps": rate, "requests": n_q, "wall_s": wall,
+            "achieved_qps": n_q / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "p99_ms_n_le_1e4": float(np.percentile(lat[small], 99)) if small.any() else None,
+            "max_ms": float(lat.max()),
+            "mean_sources_per_request": float(sizes.mean()),
+            "matches": int(nm.sum()), "matches_per_s": float(nm.sum() / wall),
+            "batches": batches, "mean_requests_per_batch": n_q / max(batches, 1),
+            "worker_busy_frac": (s1["busy_ms"] - s0["busy_ms"]) / 1e3 / wall,
+        })
+        log("rate %.0f q/s: p50 %.2f ms, p99 %.2f ms (N <= 1e4: %.2f ms), %d batches"
+            % (rate, per_rate[-1]["p50_ms"], per_rate[-1]["p99_ms"],
+               per_rate[-1]["p99_ms_n_le_1e4"] or -1, batches))
+    out = {"metric": "request latency", "unit": "ms", "n_gpus": 1, "data": "synthetic",
+           "higher_is_better": False,
+           "config": {"workload": "serving: resident %d-source catalog (galactic-plane density, "
+                                  "h = 1 arcsec), Poisson arrivals, requests N ~ logU[1, 1e5] in "
+                                  "caps of U[1, 30] arcmin, 70%% 0.5-arcsec copies of resident "
+                                  "sources, r ~ U[1, 5] arcsec" % n_res,
+                      "resident_sources": n_res, "duration_s_per_rate": args.duration},
+           "rates": per_rate}
+    res_cat.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="full_sky", choices=["full_sky", "plane", "batch"],
+    ap.add_argument("--config", default="full_sky",
+                    choices=["full_sky", "plane", "batch", "serve"],
                     help="full_sky: the 50M x 50M FP64 join (default); plane: the 200M x 200M "
                          "precision-form comparison; batch: many independent joins, batched "
-                         "against a loop of one-job calls")
+                         "against a loop of one-job calls; serve: a resident catalog "
+                         "answering Poisson request streams")
+    ap.add_argument("--rates", default="50,200,800", help="serve config: arrival rates, q/s")
+    ap.add_argument("--duration", type=float, default=10.0,
+                    help="serve config: seconds of arrivals per rate")
     ap.add_argument("--jobs", type=int, default=10_000, help="jobs of the batch config")
     ap.add_argument("--no-work-counts", action="store_true",
                     help="batch config: skip the host work counts")
@@ -325,6 +458,9 @@ def main():
         return
     if args.config == "batch":
         run_batch(args)
+        return
+    if args.config == "serve":
+        run_serve(args)
         return
 
     t0 = time.time()

@@ -32,7 +32,8 @@ typedef enum {
     XM_EINVAL = 1,    /* bad argument or bad input record (NaN, dec outside [-90,90], ra outside [0,360)) */
     XM_ENOMEM = 2,    /* device or host allocation failed */
     XM_EDEVICE = 3,   /* no usable GPU, or a HIP call failed */
-    XM_EINTERNAL = 4
+    XM_EINTERNAL = 4,
+    XM_EAGAIN = 5     /* xm_server_poll: the request is not done yet */
 } xm_status;
 
 typedef enum {
@@ -181,6 +182,51 @@ xm_status xm_result_copy(xm_ctx *ctx, const xm_result *res, int64_t *id_a, int64
 xm_status xm_result_jobs(xm_ctx *ctx, const xm_result *res, uint32_t *job, uint64_t *job_counts);
 xm_status xm_result_job_device_ptr(const xm_result *res, const uint32_t **job);
 void xm_result_destroy(xm_result *res);
+
+/* ---- serving: a resident catalog answering crossmatch requests.  No counterpart
+ * in the reference, which ran one batch of jobs and exited (BoostXmatch.cpp:
+ * 301-330); built on xm_catalog_create_segments + xm_match_batch.
+ * A server owns a worker thread that batches the pending requests continuously:
+ * it takes every waiting request (up to the admission limits: query rows per
+ * batch and free device memory), prepares them as one many-segment catalog, joins
+ * each with the resident catalog in one batched launch, and hands each request
+ * its own pairs.  A request's pairs do not depend on what it was batched with.
+ * The resident catalog must outlive the server; requests use its zone height. */
+typedef struct xm_server xm_server;
+
+typedef struct {
+    uint64_t max_batch_rows;      /* query rows per batch (0 = 1 << 24) */
+    uint32_t max_batch_requests;  /* requests per batch (0 = 4096) */
+    xm_precision precision;
+} xm_server_config;
+
+typedef struct {
+    uint64_t requests_done;
+    uint64_t batches;
+    uint64_t rows;               /* query rows joined */
+    uint64_t matches;
+    double busy_ms;              /* wall time the worker spent on batches */
+} xm_server_stats;
+
+xm_status xm_server_create(xm_ctx *ctx, const xm_catalog *resident, const xm_server_config *cfg,
+                           xm_server **out);
+/* Queue n host records to be matched with the resident catalog within radius_deg;
+ * the records are copied before the call returns.  *ticket names the request. */
+xm_status xm_server_submit(xm_server *srv, const xm_obj *records, uint64_t n, double radius_deg,
+                           uint64_t *ticket);
+/* Wait up to timeout_ms (0 = do not wait, < 0 = wait until done) for the request.
+ * XM_OK: done, *n_matches set; XM_EAGAIN: not done yet; another status: the
+ * request failed (xm_server_error gives the message) or the ticket is unknown. */
+xm_status xm_server_poll(xm_server *srv, uint64_t ticket, int32_t timeout_ms, uint64_t *n_matches);
+/* Copy a done request's pairs (query id, resident id, distance in arcsec) into
+ * host arrays of length n_matches (any may be NULL) and its submit-to-done time,
+ * then forget the request. */
+xm_status xm_server_fetch(xm_server *srv, uint64_t ticket, int64_t *id_query, int64_t *id_catalog,
+                          double *dist_arcsec, double *latency_ms);
+const char *xm_server_error(xm_server *srv, uint64_t ticket);
+xm_status xm_server_get_stats(xm_server *srv, xm_server_stats *stats);
+/* Finishes the queued requests, stops the worker and frees the server. */
+void xm_server_destroy(xm_server *srv);
 
 /* ---- work counts, computed on the host from the catalogs' sorted keys and
  * zone tables (never from device counters):

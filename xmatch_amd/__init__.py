@@ -23,10 +23,11 @@ from .synth import OBJ_DTYPE, ARCSEC, B_ID_OFFSET  # noqa: F401
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxmatch_amd.so")
 
-XM_OK, XM_EINVAL, XM_ENOMEM, XM_EDEVICE, XM_EINTERNAL = range(5)
+XM_OK, XM_EINVAL, XM_ENOMEM, XM_EDEVICE, XM_EINTERNAL, XM_EAGAIN = range(6)
 FP64_PLAIN = 0
 FP32_FILTER_FP64_RECHECK = 1
-_STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_EINTERNAL"}
+_STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_EINTERNAL",
+           5: "XM_EAGAIN"}
 
 # every symbol declared in include/xmatch.h
 EXPORTED = [
@@ -40,6 +41,8 @@ EXPORTED = [
     "xm_catalog_create_segments", "xm_catalog_create_segments_device",
     "xm_catalog_segment_count", "xm_catalog_segment_info", "xm_match_batch",
     "xm_result_jobs", "xm_result_job_device_ptr", "xm_work_counts_job",
+    "xm_server_create", "xm_server_submit", "xm_server_poll", "xm_server_fetch",
+    "xm_server_error", "xm_server_get_stats", "xm_server_destroy",
     "xm_synth_fresh", "xm_synth_copies",
 ]
 
@@ -94,6 +97,20 @@ def _job_array(jobs):
         arr[i].seg_b = int(b)
         arr[i].radius_deg = float(r)
     return arr
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch_rows", ctypes.c_uint64), ("max_batch_requests", ctypes.c_uint32),
+                ("precision", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("rows", ctypes.c_uint64), ("matches", ctypes.c_uint64),
+                ("busy_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Shape(ctypes.Structure):
@@ -170,6 +187,13 @@ def lib():
             "xm_result_job_device_ptr": ([vp, pvp], ctypes.c_int),
             "xm_work_counts_job": ([vp, vp, vp, ctypes.POINTER(Job), ctypes.POINTER(u64),
                                     ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_server_create": ([vp, vp, ctypes.POINTER(ServerConfig), pvp], ctypes.c_int),
+            "xm_server_submit": ([vp, vp, u64, dbl, ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_server_poll": ([vp, u64, ctypes.c_int32, ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_server_fetch": ([vp, u64, vp, vp, vp, ctypes.POINTER(dbl)], ctypes.c_int),
+            "xm_server_error": ([vp, u64], ctypes.c_char_p),
+            "xm_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
+            "xm_server_destroy": ([vp], None),
             "xm_synth_fresh": ([vp, ctypes.POINTER(Shape), u64, u64, u64, ctypes.c_int64, vp],
                                ctypes.c_int),
             "xm_synth_copies": ([vp, vp, u64, dbl, dbl, ctypes.POINTER(Shape), u64, u64, u64,
@@ -208,7 +232,8 @@ class Context:
     def close(self):
         """Destroy the context after every catalog and result made from it."""
         if self._h:
-            for c in list(self._children):
+            # servers first: their workers use the catalogs
+            for c in sorted(list(self._children), key=lambda c: not isinstance(c, Server)):
                 c.close()
             lib().xm_ctx_destroy(self._h)
             self._h = ctypes.c_void_p()
@@ -506,3 +531,77 @@ class Result:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """A resident catalog answering crossmatch requests with continuous batching
+    (xm_server_*).  submit() returns a ticket; result(ticket) waits and returns
+    (id_query, id_catalog, dist_arcsec, latency_ms)."""
+
+    def __init__(self, ctx, resident, max_batch_rows=0, max_batch_requests=0,
+                 precision=FP64_PLAIN):
+        self.ctx = ctx
+        self.resident = resident  # kept alive for the server's lifetime
+        cfg = ServerConfig(max_batch_rows, max_batch_requests, precision)
+        self._h = ctypes.c_void_p()
+        ctx._check(lib().xm_server_create(ctx.handle, resident.handle, ctypes.byref(cfg),
+                                          ctypes.byref(self._h)))
+        ctx._children.add(self)
+
+    def submit(self, records, radius_deg):
+        rec = np.ascontiguousarray(records)
+        if rec.dtype != OBJ_DTYPE:
+            raise TypeError("records must have dtype %r" % (OBJ_DTYPE,))
+        t = ctypes.c_uint64()
+        st = lib().xm_server_submit(self._h, _ptr(rec) if rec.size else None, rec.size,
+                                    radius_deg, ctypes.byref(t))
+        if st != XM_OK:
+            raise XmatchError(st, "request rejected (bad radius or records)")
+        return t.value
+
+    def poll(self, ticket, timeout_ms=0):
+        """Pair count when done, None while pending."""
+        n = ctypes.c_uint64()
+        st = lib().xm_server_poll(self._h, ticket, int(timeout_ms), ctypes.byref(n))
+        if st == XM_EAGAIN:
+            return None
+        if st != XM_OK:
+            raise XmatchError(st, lib().xm_server_error(self._h, ticket).decode())
+        return n.value
+
+    def result(self, ticket, timeout_ms=-1):
+        n = self.poll(ticket, timeout_ms)
+        if n is None:
+            return None
+        ida = np.empty(n, np.int64)
+        idb = np.empty(n, np.int64)
+        dist = np.empty(n, np.float64)
+        lat = ctypes.c_double()
+        st = lib().xm_server_fetch(self._h, ticket, _ptr(ida), _ptr(idb), _ptr(dist),
+                                   ctypes.byref(lat))
+        if st != XM_OK:
+            raise XmatchError(st, "fetch failed")
+        return ida, idb, dist, lat.value
+
+    def stats(self):
+        s = ServerStats()
+        self.ctx._check(lib().xm_server_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().xm_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
