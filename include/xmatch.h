@@ -105,6 +105,8 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  * Results never depend on the knobs; only the work done does. */
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
+/* Free and total device memory; blocks the context caches for reuse count as free. */
+xm_status xm_ctx_mem_info(xm_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 /* Wait for all device work of the context. */
 xm_status xm_ctx_synchronize(xm_ctx *ctx);
 /* The context keeps the device memory of finished catalogs, results and scratch
@@ -115,6 +117,10 @@ xm_status xm_ctx_release_memory(xm_ctx *ctx, uint64_t *cached_bytes);
  * records on the device). */
 xm_status xm_device_alloc(xm_ctx *ctx, size_t bytes, void **dptr);
 xm_status xm_device_free(xm_ctx *ctx, void *dptr);
+/* Page-locked host memory (hipHostMalloc): copies from it reach the device at
+ * full link speed.  For callers that stream records from files. */
+xm_status xm_host_alloc(xm_ctx *ctx, size_t bytes, void **hptr);
+xm_status xm_host_free(xm_ctx *ctx, void *hptr);
 xm_status xm_memcpy_h2d(xm_ctx *ctx, void *dst, const void *src, size_t bytes);
 xm_status xm_memcpy_d2h(xm_ctx *ctx, void *dst, const void *src, size_t bytes);
 

@@ -35,6 +35,7 @@ EXPORTED = [
     "xm_ctx_set_timing", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
     "xm_ctx_set_param", "xm_ctx_get_param", "xm_ctx_release_memory",
     "xm_device_alloc", "xm_device_free", "xm_memcpy_h2d", "xm_memcpy_d2h",
+    "xm_host_alloc", "xm_host_free", "xm_ctx_mem_info",
     "xm_catalog_create", "xm_catalog_create_device", "xm_catalog_destroy", "xm_catalog_info",
     "xm_catalog_export", "xm_match", "xm_result_count", "xm_result_device_ptrs",
     "xm_result_copy", "xm_result_destroy", "xm_work_counts",
@@ -159,6 +160,9 @@ def lib():
             "xm_device_alloc": ([vp, ctypes.c_size_t, pvp], ctypes.c_int),
             "xm_device_free": ([vp, vp], ctypes.c_int),
             "xm_memcpy_h2d": ([vp, vp, vp, ctypes.c_size_t], ctypes.c_int),
+            "xm_host_alloc": ([vp, ctypes.c_size_t, pvp], ctypes.c_int),
+            "xm_ctx_mem_info": ([vp, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+            "xm_host_free": ([vp, vp], ctypes.c_int),
             "xm_memcpy_d2h": ([vp, vp, vp, ctypes.c_size_t], ctypes.c_int),
             "xm_catalog_create": ([vp, vp, u64, dbl, pvp], ctypes.c_int),
             "xm_catalog_create_device": ([vp, vp, u64, dbl, pvp], ctypes.c_int),

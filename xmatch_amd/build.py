@@ -26,19 +26,34 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=()):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + list(extra) + srcs + ["-o", tmp]
+CLI = os.path.join(HERE, "xmatch")
+CXX = os.environ.get("CXX", "g++")
+# the command-line tool is host code over the C ABI only: plain g++, linked to the
+# library next to it
+CLI_FLAGS = ["-O2", "-std=c++17", "-Wall", "-pthread", "-I" + os.path.join(HERE, "..", "include")]
+
+
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % LIB)
+        raise RuntimeError("%s failed building %s" % (cmd[0], what))
+
+
+def build(verbose=False, extra=()):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + FLAGS + list(extra) + srcs + ["-o", tmp], LIB, verbose)
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    _run([CXX] + CLI_FLAGS + [os.path.join(CSRC, "xm_cli.cpp"), "-o", tmp, "-L" + HERE,
+                              "-lxmatch_amd", "-Wl,-rpath,$ORIGIN"], CLI, verbose)
+    os.replace(tmp, CLI)
     return LIB
 
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(CLI)

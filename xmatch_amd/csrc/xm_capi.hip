@@ -606,6 +606,19 @@ xm_status xm_ctx_release_memory(xm_ctx* ctx, uint64_t* cached_bytes) {
     });
 }
 
+xm_status xm_ctx_mem_info(xm_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        size_t f = 0, t = 0;
+        XM_HIP(hipMemGetInfo(&f, &t));
+        size_t cached_free = 0;
+        for (auto& fb : ctx->free_blocks) cached_free += fb.first;
+        if (free_bytes) *free_bytes = uint64_t(f) + cached_free;
+        if (total_bytes) *total_bytes = uint64_t(t);
+    });
+}
+
 xm_status xm_ctx_synchronize(xm_ctx* ctx) {
     if (!ctx) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -625,6 +638,26 @@ xm_status xm_device_free(xm_ctx* ctx, void* dptr) {
     if (!ctx) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     return guarded(ctx, [&] { dfree(ctx, dptr); });
+}
+
+xm_status xm_host_alloc(xm_ctx* ctx, size_t bytes, void** hptr) {
+    if (!ctx || !hptr) return XM_EINVAL;
+    *hptr = nullptr;
+    return guarded(ctx, [&] {
+        const hipError_t e = hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            throw OutOfMemory{"pinned host allocation of " + std::to_string(bytes) + " bytes failed"};
+        }
+        XM_HIP(e);
+    });
+}
+
+xm_status xm_host_free(xm_ctx* ctx, void* hptr) {
+    if (!ctx) return XM_EINVAL;
+    return guarded(ctx, [&] {
+        if (hptr) XM_HIP(hipHostFree(hptr));
+    });
 }
 
 xm_status xm_memcpy_h2d(xm_ctx* ctx, void* dst, const void* src, size_t bytes) {
