@@ -233,19 +233,21 @@ XM_HD int window_intervals(int32_t zb, uint64_t q_lo, uint64_t q_hi, int64_t aq,
     return 1;
 }
 
-// Join mode of one A chunk.  Dense: every B row of the window is tested against
-// every row of the chunk (one wave instruction per B row).  Sparse: each B row is
-// tested only against the chunk rows within +-aq of its ra (binary search on the
-// chunk's ra quanta).  Dense is used for the pole rule, for windows that cover the
-// circle, and when a B row is expected to meet at least dense_min chunk rows:
-// 64 (2aq+1) / (span + 2aq + 1) >= dense_min.
-XM_HD bool chunk_is_dense(uint64_t q_lo, uint64_t q_hi, int64_t aq, uint32_t dense_min) {
+// Join mode of one A chunk of cnt rows spanning ra quanta [q_lo, q_hi].  Dense:
+// every B row of the window is tested against every row of the chunk (one wave
+// instruction per B row and 64 chunk rows).  Sparse: each B row is tested only
+// against the chunk rows within +-aq of its ra (binary search on the chunk's ra
+// quanta).  Dense is used for the pole rule, for windows that cover the circle,
+// and when a B row is expected to meet at least dense_min chunk rows:
+// cnt (2aq+1) / (span + 2aq + 1) >= dense_min.
+XM_HD bool chunk_is_dense(uint64_t q_lo, uint64_t q_hi, int64_t aq, uint32_t dense_min,
+                          uint32_t cnt) {
     if (aq == WHOLE_ZONE) return true;
     const int64_t full = int64_t(uint64_t(1) << RA_BITS);
     const int64_t w = 2 * aq + 1;
     const int64_t span = int64_t(q_hi) - int64_t(q_lo);
     if (span + w >= full) return true;
-    return 64 * w >= int64_t(dense_min) * (span + w);
+    return int64_t(cnt) * w >= int64_t(dense_min) * (span + w);
 }
 
 }  // namespace xm

@@ -113,7 +113,7 @@ struct xm_result {
 namespace xm {
 
 constexpr int WAVE = 64;
-constexpr int CHUNK_ROWS = 64;   // A rows per wavefront in the pair kernel
+constexpr int CHUNK_ROWS = 256;  // A rows per wavefront in the pair kernel
 
 // One catalog row on the device: FP64 unit vector and id in one 32-byte line, so
 // a candidate B row is a single aligned fetch.

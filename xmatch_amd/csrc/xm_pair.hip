@@ -2,20 +2,23 @@
 //
 // Replaces the per-zone-pair launch loop of Worker::Match and xmatch_kernel
 // (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
-// into chunks of up to 64 consecutive rows of one zone; each wavefront owns one
+// into chunks of up to 256 consecutive rows of one zone; each wavefront owns one
 // chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
 // alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  Each
 // chunk is joined in one of two modes, chosen per chunk (chunk_is_dense):
 //
-//  sparse  the chunk's A unit vectors and ra quanta are staged in LDS; each lane
-//          takes one B row, checks it against the window, binary-searches the
-//          chunk for the A rows within +-alpha of its ra and tests only those.  At
-//          zone height = radius a zone holds a few hundred sources over 360 deg, so
-//          a chunk spans tens of degrees while alpha is arcseconds: a dense test of
-//          every (A, B) pair of the window would almost always be wasted.
-//  dense   (pole rule, windows covering the circle, clustered data) one A row per
-//          lane in registers; the window's B rows are walked in wave-uniform order
-//          as wide scalar (SGPR) loads, every lane testing its A row against each.
+//  sparse  the chunk's ra quanta are staged in LDS; each lane takes one B row,
+//          checks it against the window, binary-searches the chunk for the A rows
+//          within +-alpha of its ra and tests only those.  At zone height = radius
+//          a zone holds a few hundred sources over 360 deg, so a chunk spans most
+//          of the circle while alpha is arcseconds: a dense test of every (A, B)
+//          pair of the window would almost always be wasted.  With 256-row chunks
+//          a zone is usually one chunk, so each B row of its neighbour zones is
+//          examined about once per A zone.
+//  dense   (pole rule, windows covering the circle, clustered data) 64 rows at a
+//          time, one A row per lane in registers; the window's B rows are walked
+//          in wave-uniform order as wide scalar (SGPR) loads, every lane testing
+//          its A row against each.
 //
 // Predicate (Worker.cu:88-97 with the seam bug D2 corrected), on FP64 unit vectors:
 //   d2 = fma(dz, dz, fma(dy, dy, dx*dx))  <  (2 sin(r/2))^2
@@ -335,23 +338,24 @@ __global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
 }
 
 // --------------------------------------------------------------- sparse ----
-template <typename T>
 struct SparseCtx {
     uint32_t start, cnt;
+    uint32_t half;            // search width / 2: the power of two >= cnt, halved
     int32_t aq;
-    const int32_t* s_aq;
-    const T *s_ax, *s_ay, *s_az;
-    T thr;  // keep a pair when d2 < thr
+    const int32_t* s_aq;      // ra quanta of the chunk rows, padded with INT_MAX
 };
 
 // One B row per lane (row j, key); tested against the chunk rows within +-aq of
 // its ra if it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).
+// The chunk's ra quanta are in LDS; a candidate's A unit vector is read from
+// memory (the chunk's rows are contiguous, so these reads share cache lines).
 template <bool F32>
-__device__ inline void sparse_row(const SparseCtx<typename Prec<F32>::T>& x, uint32_t j, bool valid,
-                                  uint64_t key, int ni, const int32_t* lq, const int32_t* hq,
-                                  const int32_t* shift,
-                                  const typename Prec<F32>::Row* __restrict__ brow, uint32_t lane,
-                                  Stage& st, const Out& o) {
+__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key, int ni,
+                                  const int32_t* lq, const int32_t* hq, const int32_t* shift,
+                                  const typename Prec<F32>::Row* __restrict__ arow,
+                                  const typename Prec<F32>::Row* __restrict__ brow,
+                                  typename Prec<F32>::T thr, uint32_t lane, Stage& st,
+                                  const Out& o) {
     using T = typename Prec<F32>::T;
     const int32_t q = int32_t(key & RA_MASK);
     bool in = false;
@@ -365,17 +369,16 @@ __device__ inline void sparse_row(const SparseCtx<typename Prec<F32>::T>& x, uin
     }
     if (!__ballot(in)) return;
     const int32_t lo_q = qb - x.aq, hi_q = qb + x.aq;
-    // c = number of chunk rows with quantum < lo_q (branchless search of 64)
+    // c = number of chunk rows with quantum < lo_q (branchless binary search)
     uint32_t c = 0;
-#pragma unroll
-    for (uint32_t step = WAVE / 2; step >= 1; step >>= 1)
+    for (uint32_t step = x.half; step >= 1; step >>= 1)
         if (x.s_aq[c + step - 1] < lo_q) c += step;
     if (x.s_aq[c] < lo_q) c++;
     bool have_b = false;
     T bx = 0, by = 0, bz = 0;
     for (uint32_t k = 0;; k++) {
         const uint32_t cc = c + k;
-        const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(WAVE) ? cc : 0] <= hi_q;
+        const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(CHUNK_ROWS) ? cc : 0] <= hi_q;
         if (!__ballot(act)) break;
         T d2 = 0;
         bool hit = false;
@@ -387,8 +390,9 @@ __device__ inline void sparse_row(const SparseCtx<typename Prec<F32>::T>& x, uin
                 bz = b.z;
                 have_b = true;
             }
-            d2 = chord2<T>(x.s_ax[cc], x.s_ay[cc], x.s_az[cc], bx, by, bz);
-            hit = d2 < x.thr;
+            const auto a = arow[x.start + cc];
+            d2 = chord2<T>(a.x, a.y, a.z, bx, by, bz);
+            hit = d2 < thr;
         }
         const uint64_t m = __ballot(hit);
         if (m) stage_hits<F32>(m, double(d2), x.start + cc, j, lane, st, o);
@@ -449,10 +453,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
-    __shared__ int32_t s_aq[PK_WAVES][WAVE];
-    __shared__ T s_ax[PK_WAVES][WAVE];
-    __shared__ T s_ay[PK_WAVES][WAVE];
-    __shared__ T s_az[PK_WAVES][WAVE];
+    __shared__ int32_t s_aq[PK_WAVES][CHUNK_ROWS];
 
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
@@ -476,33 +477,59 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const uint32_t cnt = uni(chunks[c].count);
     const int32_t za = int32_t(uni(uint32_t(chunks[c].zone)));
 
-    const uint32_t row = start + lane;
-    int32_t qa = INT_MAX;
-    T ax = T(10), ay = T(10), az = T(10);  // inactive lanes sit far off the sphere
-    if (lane < cnt) {
-        const auto a = arow_t[row];
-        ax = a.x;
-        ay = a.y;
-        az = a.z;
-        qa = int32_t(akey[row] & RA_MASK);
+    // the chunk's ra quanta in LDS, padded with INT_MAX to CHUNK_ROWS
+#pragma unroll
+    for (uint32_t k = 0; k < uint32_t(CHUNK_ROWS) / WAVE; k++) {
+        const uint32_t r = k * WAVE + lane;
+        s_aq[w][r] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
     }
-    const uint64_t q_lo = uint32_t(__builtin_amdgcn_readlane(qa, 0));
-    const uint64_t q_hi = uint32_t(__builtin_amdgcn_readlane(qa, int(cnt - 1)));
+    wave_lds_sync();
+    const uint64_t q_lo = uint32_t(s_aq[w][0]);
+    const uint64_t q_hi = uint32_t(s_aq[w][cnt - 1]);
     const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, za, pp.atan_sin_theta, pp.gran);
-    const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min);
+    const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min, cnt);
     const T thr = F32 ? T(pp.t_hi32) : T(pp.chord2);
-    if (!dense) {
-        s_aq[w][lane] = qa;
-        s_ax[w][lane] = ax;
-        s_ay[w][lane] = ay;
-        s_az[w][lane] = az;
-        wave_lds_sync();
-    }
-
-    const SparseCtx<T> x{start, cnt, int32_t(aq), s_aq[w], s_ax[w], s_ay[w], s_az[w], thr};
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
     const Out o = make_out(arow, brow, ro, pp, jid);
 
+    if (dense) {
+        // 64 rows at a time, one A row per lane in registers; each run of 64 joins
+        // the B rows of its own window
+        for (uint32_t sc = 0; sc < cnt; sc += WAVE) {
+            const uint32_t sub = cnt - sc < uint32_t(WAVE) ? cnt - sc : uint32_t(WAVE);
+            const uint32_t row = start + sc + lane;
+            T ax = T(10), ay = T(10), az = T(10);  // inactive lanes sit far off the sphere
+            if (lane < sub) {
+                const auto a = arow_t[row];
+                ax = a.x;
+                ay = a.y;
+                az = a.z;
+            }
+            const uint64_t s_lo = uint32_t(s_aq[w][sc]);
+            const uint64_t s_hi = uint32_t(s_aq[w][sc + sub - 1]);
+            for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
+                const int32_t zb = za + dz;
+                if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
+                const uint32_t b0 = uni(bzoff[zb - bzmin]);
+                const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
+                if (b0 >= b1) continue;
+                uint64_t lo[2], hi[2];
+                int32_t shift[2];
+                const int ni = window_intervals(zb, s_lo, s_hi, aq, lo, hi, shift);
+                for (int t = 0; t < ni; t++) {
+                    const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t] | bpre, lane);
+                    const uint32_t j1 = wave_lower_bound(bkey, j0, b1, (hi[t] | bpre) + 1u, lane);
+                    dense_range<F32>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
+                }
+            }
+        }
+        if (st.n) flush_stage<F32>(st, lane, o);
+        return;
+    }
+
+    uint32_t half = 1;
+    while (2 * half < cnt) half *= 2;
+    const SparseCtx x{start, cnt, half, int32_t(aq), s_aq[w]};
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
         if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
@@ -513,20 +540,10 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
         int32_t shift[2], lq[2], hq[2];
         const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
         for (int t = 0; t < ni; t++) {
-            lo[t] |= bpre;
-            hi[t] |= bpre;
-        }
-        if (dense) {
-            for (int t = 0; t < ni; t++) {
-                const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
-                const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
-                dense_range<F32>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
-            }
-            continue;
-        }
-        for (int t = 0; t < ni; t++) {
             lq[t] = int32_t(lo[t] & RA_MASK);
             hq[t] = int32_t(hi[t] & RA_MASK);
+            lo[t] |= bpre;
+            hi[t] |= bpre;
         }
         if (b1 - b0 <= ONE_PASS_ROWS) {
             // the whole zone in one round trip; rows outside the window drop out
@@ -540,7 +557,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 if (b0 + k * WAVE >= b1) break;
                 const uint32_t j = b0 + k * WAVE + lane;
-                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, brow_t, lane, st, o);
+                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, arow_t, brow_t, thr,
+                                lane, st, o);
             }
         } else {
             for (int t = 0; t < ni; t++) {
@@ -550,8 +568,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
                     const uint32_t j = jb + lane;
                     const bool valid = j < j1;
                     const uint64_t key = valid ? bkey[j] : 0;
-                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], brow_t, lane,
-                                    st, o);
+                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], arow_t,
+                                    brow_t, thr, lane, st, o);
                 }
             }
         }

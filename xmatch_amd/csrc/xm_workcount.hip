@@ -154,17 +154,29 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
                     for (uint64_t s = a0; s < a1; s += CHUNK_ROWS) {
                         const uint64_t cnt = std::min<uint64_t>(CHUNK_ROWS, a1 - s);
                         const uint64_t q_lo = ka[s] & RA_MASK, q_hi = ka[s + cnt - 1] & RA_MASK;
-                        const bool dense = chunk_is_dense(q_lo, q_hi, aq, ctx->dense_min);
+                        const bool dense =
+                            chunk_is_dense(q_lo, q_hi, aq, ctx->dense_min, uint32_t(cnt));
                         uint64_t lo[2], hi[2];
                         int32_t shift[2];
+                        if (dense) {
+                            // runs of 64 rows, each over its own window
+                            for (uint64_t sc = s; sc < s + cnt; sc += 64) {
+                                const uint64_t sub = std::min<uint64_t>(64, s + cnt - sc);
+                                const int ni = window_intervals(int32_t(z2), ka[sc] & RA_MASK,
+                                                                ka[sc + sub - 1] & RA_MASK, aq, lo,
+                                                                hi, shift);
+                                for (int t2 = 0; t2 < ni; t2++) {
+                                    const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
+                                    const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                                    s_tile += sub * (j1 - j0);
+                                }
+                            }
+                            continue;
+                        }
                         const int ni = window_intervals(int32_t(z2), q_lo, q_hi, aq, lo, hi, shift);
                         for (int t2 = 0; t2 < ni; t2++) {
                             const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
                             const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
-                            if (dense) {
-                                s_tile += cnt * (j1 - j0);
-                                continue;
-                            }
                             // sparse: chunk rows within +-aq of each B row's quantum
                             int32_t qs[CHUNK_ROWS];
                             for (uint64_t k = 0; k < cnt; k++) qs[k] = int32_t(ka[s + k] & RA_MASK);
