@@ -887,36 +887,38 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
         }
         cap = std::max<uint64_t>(cap, 1024);
         for (auto& e : ev) XM_HIP(hipEventCreate(&e));
-        // Hits go to n_regions append regions as (A row, B row[, job]); blocks spread
-        // over them round-robin.  The first run gives the regions 3/4 of the slots
-        // and keeps 1/4 as a shared spill pool, so uneven regions do not overflow
-        // while the total fits.  A one-block scan sizes the parts; one host round
-        // trip reads the total; k_compact then gathers ids and distances into a
-        // result of exactly the total.  If the pool overflowed, the re-run gives
-        // every region exactly its count from the first run (the block -> region
-        // assignment and each block's hits do not change), so it cannot overflow.
+        // Hits go to n_regions append regions as (A row, B row[, job]); wavefronts
+        // spread over them round-robin.  The first run gives the regions 3/4 of the slots
+        // and keeps 1/4 as 64 spill pools (region r spills into pool r % 64, each
+        // pool with its own counter line), so uneven regions do not overflow while
+        // the total fits.  A one-block scan sizes the parts; one host round trip
+        // reads the total; k_compact then gathers ids and distances into a result
+        // of exactly the total.  If a pool overflowed, the re-run gives every region
+        // exactly its count from the first run (the wave -> region assignment and
+        // each wave's hits do not change), so it cannot overflow.
         ro.n_regions = std::max<uint32_t>(
             1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(std::max<uint64_t>(jt.max_waves, 1))));
         const uint32_t R = ro.n_regions;
+        const uint32_t P = SPILL_POOLS;
         ro.counters = static_cast<uint64_t*>(
-            dalloc(ctx, size_t(R + 1) * REGION_STRIDE * sizeof(uint64_t)));
-        // scratch: prefix[R + 2], tail[4], base[R + 1], cap[R], next_base[R + 1], next_cap[R]
-        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(R) * 5 + 8) * sizeof(uint64_t)));
+            dalloc(ctx, size_t(R + P) * REGION_STRIDE * sizeof(uint64_t)));
+        // scratch: prefix[R + P + 1], tail[4], base[R + P], cap[R], next_base[R + P], next_cap[R]
+        scratch = static_cast<uint64_t*>(dalloc(ctx, (size_t(R) * 5 + 3 * P + 8) * sizeof(uint64_t)));
         uint64_t* prefix = scratch;
-        uint64_t* tail = prefix + R + 2;
+        uint64_t* tail = prefix + R + P + 1;
         uint64_t* base = tail + 4;
-        uint64_t* capa = base + R + 1;
+        uint64_t* capa = base + R + P;
         uint64_t* next_base = capa + R;
-        uint64_t* next_cap = next_base + R + 1;
+        uint64_t* next_cap = next_base + R + P;
         if (batch) {
             ro.n_jobs = uint32_t(n_jobs);
             ro.job_count = static_cast<unsigned long long*>(dalloc(ctx, size_t(JOB_SPREAD) * n_jobs * 8));
             job_sums = static_cast<uint64_t*>(dalloc(ctx, n_jobs * 8));
         }
         const uint64_t cap_each = std::max<uint64_t>(1, cap * 3 / 4 / R);
-        launch_region_init(ctx, R, cap_each, base, capa);
-        uint64_t slots = std::max<uint64_t>(cap, cap_each * R + 1);
-        ro.spill_cap = slots - cap_each * R;
+        ro.spill_cap = std::max<uint64_t>(1, (cap - std::min(cap, cap_each * R)) / P);
+        launch_region_init(ctx, R, cap_each, ro.spill_cap, base, capa);
+        uint64_t slots = cap_each * R + ro.spill_cap * P;
         uint64_t* pin = static_cast<uint64_t*>(ctx->pinned);
         uint64_t total = 0;
         for (int attempt = 0;; attempt++) {
@@ -937,7 +939,7 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
                 launch_fill_u64(ctx, reinterpret_cast<uint64_t*>(ro.job_count),
                                 size_t(JOB_SPREAD) * n_jobs, 0);
             }
-            launch_fill_u64(ctx, ro.counters, size_t(R + 1) * REGION_STRIDE, 0);
+            launch_fill_u64(ctx, ro.counters, size_t(R + P) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev[0], ctx->stream));
             launch_pair(ctx, A, b, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
             launch_region_scan(ctx, ro, prefix, tail, next_base, next_cap);

@@ -209,22 +209,24 @@ struct PairParams {
     int32_t n_zones;
     uint32_t dense_min;     // chunk mode rule, see chunk_is_dense()
 };
-// Output of a join.  Hits are appended to n_regions regions; block b appends to
-// region b % n_regions through counters[r * REGION_STRIDE] (one 128-byte line per
-// counter, so the waves of a join never queue on one address).  Region r owns
+// Output of a join.  Hits are appended to n_regions regions; wavefront g appends
+// to region g % n_regions through counters[r * REGION_STRIDE] (one 128-byte line
+// per counter, so the waves of a join never queue on one address).  Region r owns
 // slots [base[r], base[r] + cap[r]); a flush that runs past its region's end puts
-// the rest in the spill pool [base[n_regions], + spill_cap) through the spill
-// counter counters[n_regions * REGION_STRIDE].  Counters count every hit, so when
-// the spill pool overflows the caller re-runs with each region sized to its count.
+// the rest in spill pool p = r % SPILL_POOLS, slots [base[n_regions + p], +
+// spill_cap), through that pool's counter counters[(n_regions + p) * REGION_STRIDE].
+// Counters count every hit, so when a pool overflows the caller re-runs with each
+// region sized to its count.
 constexpr uint32_t MAX_REGIONS = 1024;
+constexpr uint32_t SPILL_POOLS = 64;
 constexpr uint32_t REGION_STRIDE = 16;
-constexpr uint32_t JOB_SPREAD = 16;  // partial hit counters per job, on separate lines
+constexpr uint32_t JOB_SPREAD = 64;  // partial hit counters per job, on separate lines
 struct RegionOut {
-    uint64_t* counters;      // [(n_regions + 1) * REGION_STRIDE]
+    uint64_t* counters;      // [(n_regions + SPILL_POOLS) * REGION_STRIDE]
     uint32_t n_regions;
-    const uint64_t* base;    // [n_regions + 1]: first slot of each region, then of the pool
+    const uint64_t* base;    // [n_regions + SPILL_POOLS]: first slot of each region, then pool
     const uint64_t* cap;     // [n_regions]: slots of each region
-    uint64_t spill_cap;
+    uint64_t spill_cap;      // slots of each pool
     // wide slots hold (idA, idB, chord^2), written while the rows are in cache;
     // narrow slots hold (A row, B row), 8 bytes instead of 24, for very large outputs
     bool wide;
@@ -261,13 +263,14 @@ void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Jo
                  const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
                  const RegionOut& ro);
 void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
-// Regions of cap_each slots back to back, the spill pool after them.
-void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t* base,
-                        uint64_t* cap);
-// prefix[r] (r <= n_regions) = kept hits of the regions before r, so the pool's kept
-// hits start at prefix[n_regions]; tail[0] = all hits, tail[1] = spill demand,
-// tail[2] = FP32 band hits, tail[3] = largest |chord_32 - chord_64| (double bits);
-// next_base / next_cap = regions sized to their counts, for a re-run.
+// Regions of cap_each slots back to back, then SPILL_POOLS pools of pool_cap.
+void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t pool_cap,
+                        uint64_t* base, uint64_t* cap);
+// Parts = the regions, then the pools.  prefix[k] (k <= parts) = kept hits of the
+// parts before k; tail[0] = all hits, tail[1] = the largest pool demand, tail[2] =
+// FP32 band hits, tail[3] = largest |chord_32 - chord_64| (double bits);
+// next_base / next_cap = regions sized to their counts and empty pools, for a
+// re-run.
 void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint64_t* tail,
                         uint64_t* next_base, uint64_t* next_cap);
 // job_counts[j] = sum of the JOB_SPREAD partial counts of job j.

@@ -145,17 +145,20 @@ struct Out {
     uint32_t jid;
 };
 
+// Wave g appends to region g % n_regions: consecutive chunks of one job, which
+// carry similar numbers of hits, are dealt to all regions in turn.
 __device__ inline Out make_out(const Row64* arow, const Row64* brow, const RegionOut& ro,
-                               const PairParams& pp, uint32_t jid) {
-    const uint32_t r = blockIdx.x % ro.n_regions;
+                               const PairParams& pp, uint32_t jid, uint32_t g) {
+    const uint32_t r = g % ro.n_regions;
     Out o;
     o.arow = arow;
     o.brow = brow;
     o.counter = (unsigned long long*)(ro.counters + size_t(r) * REGION_STRIDE);
-    o.spill = (unsigned long long*)(ro.counters + size_t(ro.n_regions) * REGION_STRIDE);
+    const uint32_t pool = ro.n_regions + r % SPILL_POOLS;
+    o.spill = (unsigned long long*)(ro.counters + size_t(pool) * REGION_STRIDE);
     o.base = ro.base[r];
     o.cap = ro.cap[r];
-    o.spill_base = ro.base[ro.n_regions];
+    o.spill_base = ro.base[pool];
     o.spill_cap = ro.spill_cap;
     o.wide = ro.wide;
     o.id_a = ro.id_a;
@@ -165,7 +168,7 @@ __device__ inline Out make_out(const Row64* arow, const Row64* brow, const Regio
     o.rb = ro.rb;
     o.job = ro.job;
     o.job_count = ro.job_count
-                      ? ro.job_count + size_t(blockIdx.x % JOB_SPREAD) * ro.n_jobs + jid
+                      ? ro.job_count + size_t(g % JOB_SPREAD) * ro.n_jobs + jid
                       : nullptr;
     o.thr = pp.chord2;
     o.t_lo = double(pp.t_lo32);
@@ -285,8 +288,8 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-// Output row i of the packed result comes from part r (a region, or the spill
-// pool as part n_regions; prefix[r] <= i < prefix[r+1]) at slot base[r] + i -
+// Output row i of the packed result comes from part r (a region, or a spill pool
+// as parts n_regions ..; prefix[r] <= i < prefix[r+1]) at slot base[r] + i -
 // prefix[r].  Wide slots (wa != null) hold the ids and chord^2; for narrow slots
 // the pair's rows give the ids and the FP64 chord -- the same chord2<double> of the
 // same two unit vectors the pair kernel tested, so the bits agree.  chord^2 turns
@@ -307,8 +310,8 @@ __global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
                                                  int64_t* __restrict__ id_b,
                                                  double* __restrict__ dist,
                                                  uint32_t* __restrict__ job) {
-    __shared__ uint64_t s_pre[MAX_REGIONS + 1];
-    __shared__ uint64_t s_base[MAX_REGIONS + 1];
+    __shared__ uint64_t s_pre[MAX_REGIONS + SPILL_POOLS];
+    __shared__ uint64_t s_base[MAX_REGIONS + SPILL_POOLS];
     for (uint32_t r = threadIdx.x; r < n_parts; r += 256) {
         s_pre[r] = prefix[r];
         s_base[r] = base[r];
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min, cnt);
     const T thr = F32 ? T(pp.t_hi32) : T(pp.chord2);
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
-    const Out o = make_out(arow, brow, ro, pp, jid);
+    const Out o = make_out(arow, brow, ro, pp, jid, g);
 
     if (dense) {
         // 64 rows at a time, one A row per lane in registers; each run of 64 joins
@@ -577,11 +580,12 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     if (st.n) flush_stage<F32>(st, lane, o);
 }
 
-// One block.  kept_r = min(count_r, cap_r) for the regions and min(spill, spill_cap)
-// for the pool (part n_regions); prefix = exclusive scan of kept over the parts;
-// tail[0] = sum of count_r (every hit), tail[1] = spill demand, tail[2] = band hits,
-// tail[3] = the largest chord error (double bits); next_base = exclusive scan of
-// count_r (the pool after them, empty), next_cap = count_r.
+// One block.  Parts: the n_regions regions (kept = min(count, cap)), then the
+// SPILL_POOLS pools (kept = min(demand, spill_cap)); prefix = exclusive scan of
+// kept over the parts; tail[0] = sum of the region counts (every hit), tail[1] =
+// the largest pool demand, tail[2] = band hits, tail[3] = the largest chord error
+// (double bits); next_base = exclusive scan of the region counts, then the pools
+// (empty, at the total), next_cap = the region counts.
 __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict__ counters,
                                                       uint32_t n_regions,
                                                       const uint64_t* __restrict__ cap,
@@ -594,6 +598,8 @@ __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict
     __shared__ uint64_t s_cnt[1024];
     __shared__ uint64_t s_band[1024];
     __shared__ uint64_t s_err[1024];
+    __shared__ uint64_t s_pool[SPILL_POOLS];
+    __shared__ uint64_t s_pmax[SPILL_POOLS];
     const uint32_t t = threadIdx.x;
     const bool v = t < n_regions;
     const uint64_t c = v ? counters[size_t(t) * REGION_STRIDE] : 0;
@@ -602,6 +608,11 @@ __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict
     s_cnt[t] = c;
     s_band[t] = v ? counters[size_t(t) * REGION_STRIDE + 1] : 0;
     s_err[t] = v ? counters[size_t(t) * REGION_STRIDE + 2] : 0;
+    if (t < SPILL_POOLS) {
+        const uint64_t d = counters[size_t(n_regions + t) * REGION_STRIDE];
+        s_pool[t] = d < spill_cap ? d : spill_cap;
+        s_pmax[t] = d;
+    }
     __syncthreads();
     // Hillis-Steele inclusive scans of kept and of count; tree reductions of the rest
     for (uint32_t d = 1; d < 1024; d <<= 1) {
@@ -622,28 +633,38 @@ __global__ __launch_bounds__(1024) void k_region_scan(const uint64_t* __restrict
         if (t < d) {
             if (s_err[t + d] > s_err[t]) s_err[t] = s_err[t + d];
             s_band[t] += s_band[t + d];
+            if (t < SPILL_POOLS && t + d < SPILL_POOLS && s_pmax[t + d] > s_pmax[t])
+                s_pmax[t] = s_pmax[t + d];
         }
         __syncthreads();
     }
     if (t == 0) {
-        const uint64_t spill = counters[size_t(n_regions) * REGION_STRIDE];
-        prefix[n_regions] = s_kept[1023];
-        prefix[n_regions + 1] = s_kept[1023] + (spill < spill_cap ? spill : spill_cap);
-        next_base[n_regions] = s_cnt[1023];
+        uint64_t run = s_kept[1023];
+        for (uint32_t p = 0; p < SPILL_POOLS; p++) {
+            prefix[n_regions + p] = run;
+            run += s_pool[p];
+            next_base[n_regions + p] = s_cnt[1023];
+        }
+        prefix[n_regions + SPILL_POOLS] = run;
         tail[0] = s_cnt[1023];
-        tail[1] = spill;
+        tail[1] = s_pmax[0];
         tail[2] = s_band[0];
         tail[3] = s_err[0];
     }
 }
 
 __global__ __launch_bounds__(256) void k_region_init(uint32_t n_regions, uint64_t cap_each,
+                                                     uint64_t pool_cap,
                                                      uint64_t* __restrict__ base,
                                                      uint64_t* __restrict__ cap) {
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
-    if (r > n_regions) return;
-    base[r] = uint64_t(r) * cap_each;
-    if (r < n_regions) cap[r] = cap_each;
+    if (r >= n_regions + SPILL_POOLS) return;
+    if (r < n_regions) {
+        base[r] = uint64_t(r) * cap_each;
+        cap[r] = cap_each;
+    } else {
+        base[r] = uint64_t(n_regions) * cap_each + uint64_t(r - n_regions) * pool_cap;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_job_sum(const unsigned long long* __restrict__ partial,
@@ -687,7 +708,7 @@ void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, ui
     if (total == 0) return;
     launch_timed(ctx, "compact", [&] {
         hipLaunchKernelGGL(k_compact, dim3(uint32_t((total + 255) / 256)), dim3(256), 0,
-                           ctx->stream, ro.n_regions + 1, prefix, ro.base, total,
+                           ctx->stream, ro.n_regions + SPILL_POOLS, prefix, ro.base, total,
                            (const int64_t*)(ro.wide ? ro.id_a : nullptr),
                            (const int64_t*)ro.id_b, (const double*)ro.d2,
                            (const uint32_t*)ro.ra, (const uint32_t*)ro.rb,
@@ -697,11 +718,11 @@ void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, ui
     });
 }
 
-void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t* base,
-                        uint64_t* cap) {
+void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t pool_cap,
+                        uint64_t* base, uint64_t* cap) {
     launch_timed(ctx, "fill", [&] {
-        hipLaunchKernelGGL(k_region_init, dim3((n_regions + 1 + 255) / 256), dim3(256), 0,
-                           ctx->stream, n_regions, cap_each, base, cap);
+        hipLaunchKernelGGL(k_region_init, dim3((n_regions + SPILL_POOLS + 255) / 256), dim3(256),
+                           0, ctx->stream, n_regions, cap_each, pool_cap, base, cap);
     });
 }
 
