@@ -876,14 +876,20 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
             for (uint64_t j = 0; j < n_jobs; j++)
                 cap += 2 * std::max<uint64_t>(A->seg[jobs[j].seg_a].n, b->seg[jobs[j].seg_b].n);
         }
-        // a default that free memory cannot hold is cut to a quarter of it (as wide
-        // slots); a join that needs more re-runs once at its exact size
+        // a default that free memory cannot hold is cut to a third of it (the
+        // context's cached blocks count as free; slots narrow past the wide limit);
+        // a join that needs more re-runs once at its exact size
         if (!capacity_hint) {
             size_t free_b = 0, total_b = 0;
-            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-                cap = std::min<uint64_t>(cap, free_b / 4 / 24);
-            else
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+                uint64_t avail = free_b;
+                for (auto& fb : ctx->free_blocks) avail += fb.first;
+                const uint64_t job_b = batch ? 4 : 0;
+                const uint64_t slot_b = (cap * 24 <= WIDE_STAGING_BYTES ? 24 : 8) + job_b;
+                cap = std::min<uint64_t>(cap, avail / 3 / slot_b);
+            } else {
                 (void)hipGetLastError();
+            }
         }
         cap = std::max<uint64_t>(cap, 1024);
         for (auto& e : ev) XM_HIP(hipEventCreate(&e));
