@@ -533,7 +533,7 @@ def main():
                   for k in ("radix_count", "radix_scatter", "scan_reduce", "scan_top",
                             "scan_apply")) / args.steps
     passes = ktimes.get("radix_scatter", {}).get("launches", 0) / max(1, args.steps) / 2
-    sort_bytes = 2 * args.n * (8 + passes * 32)  # count 8 + scatter 24 per pass, both catalogs
+    sort_bytes = 2 * args.n * passes * 24  # per pass: count 8, scatter 8 + 8; both catalogs
 
     line = {
         "metric": "pair-tests/s", "value": value, "unit": "pair-tests/s", "n_gpus": 1,

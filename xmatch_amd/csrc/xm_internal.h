@@ -161,11 +161,11 @@ inline void launch_timed(xm_ctx* ctx, const char* name, F&& f) {
 // ---- scan / sort (xm_sort.hip)
 // Exclusive prefix sum of n uint32 values in place.
 void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
-// Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), 8-bit digits,
-// skipping the digits flagged in skip_mask_digits.  Buffers are ping-ponged; on
-// return *keys points at the sorted keys and *keys_alt at scratch.
+// Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), in 8-bit digits
+// from lo_bit up; bits from hi_bit up must be equal in all keys.  Buffers are
+// ping-ponged; on return *keys points at the sorted keys and *keys_alt at scratch.
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit, uint64_t skip_mask_digits);
+                     int hi_bit);
 
 // ---- catalog kernels (xm_catalog.hip)
 // One-segment build: packed sort words (pack_word) of the records, their zone

@@ -280,14 +280,15 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
 }
 
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit, uint64_t skip_mask_digits) {
+                     int hi_bit) {
     if (n <= 1) return;
     if (n > 0xffffffffull) throw InvalidArg{"radix sort supports fewer than 2^32 keys"};
+    if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
     uint32_t* counts = static_cast<uint32_t*>(dalloc(ctx, size_t(256) * nblocks * sizeof(uint32_t)));
-    for (int d = lo_bit / 8; d * 8 < hi_bit && d < 8; d++) {
-        if ((skip_mask_digits >> d) & 1u) continue;
-        const int shift = d * 8;
+    // 8-bit digits starting at lo_bit (the last one may reach past hi_bit into bits
+    // that are equal in every key)
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         launch_timed(ctx, "radix_count", [&] {
             hipLaunchKernelGGL(k_radix_count, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
                                (const uint64_t*)*keys, uint32_t(n), shift, counts, nblocks);
