@@ -281,11 +281,20 @@ __device__ inline void stage_hits(uint64_t m, double d2, uint32_t ra, uint32_t r
     if (st.n > uint32_t(STAGE - WAVE)) flush_stage<F32>(st, lane, o);
 }
 
-// Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of
-// chunks so that neighbouring chunks, which read the same B zones, share an L2.
+// Blocks are dealt to the 8 XCDs round-robin.  Give each XCD runs of XCD_RUN
+// consecutive blocks, the runs dealt in turn: neighbouring chunks, which read the
+// same B zones, share an L2, and every XCD gets runs from the whole range (chunks
+// are in zone order, and a zone's cost follows its population, which varies with
+// declination: contiguous eighths of the range would leave the polar XCDs idle).
+#ifndef XCD_RUN
+#define XCD_RUN 16
+#endif
 __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
-    const uint32_t per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
-    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    constexpr uint32_t X = 8, G = XCD_RUN;
+    const uint32_t full = nb / (X * G) * (X * G);
+    if (b >= full) return b;  // the last partial round keeps its order
+    const uint32_t xcd = b % X, k = b / X;
+    return (k / G) * (X * G) + xcd * G + k % G;
 }
 
 // Output row i of the packed result comes from part r (a region, or a spill pool
