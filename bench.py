@@ -39,7 +39,7 @@ BYTES_PER_SOURCE = {
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
 }
 KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
-           "scan_apply", "gather", "zone_offsets", "chunk_counts", "chunk_fill", "fill",
+           "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
            "pair_fp64", "region_scan", "compact"]
 
 

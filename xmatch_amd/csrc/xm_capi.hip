@@ -289,19 +289,24 @@ uint64_t seg_table(const std::vector<const xm_obj*>& recs, const std::vector<uin
 
 // Zone tables of every segment from the sorted catalog keys, and the device copy
 // of the segment table with the final zone ranges.
-void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
+// Zone-offset table layout: segment s owns entries zoff_base .. zoff_base +
+// (zmax - zmin) (none when empty) and one last entry holds n.  The gather fills
+// it; an empty catalog holds only that last entry.
+void zoff_layout(xm_ctx* ctx, xm_catalog* cat) {
     uint64_t entries = 0;
     for (auto& d : cat->seg) {
         d.zoff_base = uint32_t(entries);
         if (d.n) entries += uint64_t(d.zmax - d.zmin + 1);
-        d.recs = nullptr;
     }
     cat->n_zoff = entries + 1;
+    cat->zoff = static_cast<uint32_t*>(dalloc(ctx, cat->n_zoff * sizeof(uint32_t)));
+    if (cat->n == 0) launch_fill_i32(ctx, reinterpret_cast<int32_t*>(cat->zoff), 1, 0);
+}
+void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
+    for (auto& d : cat->seg) d.recs = nullptr;
     if (!cat->seg_dev) cat->seg_dev = static_cast<SegDesc*>(dalloc(ctx, cat->seg.size() * sizeof(SegDesc)));
     XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), cat->seg.size() * sizeof(SegDesc),
                           hipMemcpyHostToDevice, ctx->stream));
-    cat->zoff = static_cast<uint32_t*>(dalloc(ctx, cat->n_zoff * sizeof(uint32_t)));
-    launch_zone_offsets(ctx, cat->key, cat->n, cat->seg_dev, cat->n_zoff, cat->zoff);
     // cat->seg stays as uploaded; the creating call synchronises before returning
 }
 
@@ -363,10 +368,12 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
     radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit);
-    dfree(ctx, words_alt);
+    zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
-    launch_gather(ctx, d.recs, words, n, pl, static_cast<Row64*>(cat->row));
-    cat->key = words;
+    launch_gather(ctx, d.recs, words, n, pl, zmin, static_cast<Row64*>(cat->row), cat->zoff,
+                  cat->n_zoff, words_alt);
+    dfree(ctx, words);
+    cat->key = words_alt;
 }
 
 // Many segments: pass 1 finds every segment's zone range, the host picks each
@@ -423,16 +430,18 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
         content = std::max(content, idx_bits + d.ra_bits + d.zone_bits);
     }
     const int seg_shift = seg_bits ? content : 64;
+    zoff_layout(ctx, cat);
     XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice,
                           ctx->stream));
     uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
     uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
     launch_seg_words(ctx, cat->seg_dev, S, n, seg_shift, words);
     radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits);
-    dfree(ctx, words_alt);
     cat->row = dalloc(ctx, n * sizeof(Row64));
-    launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row));
-    cat->key = words;
+    launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row),
+                      cat->zoff, cat->n_zoff, words_alt);
+    dfree(ctx, words);
+    cat->key = words_alt;
     sync(ctx);  // the host segment table is rewritten next; its uploads must be done
 }
 
@@ -441,7 +450,8 @@ void catalog_build(xm_ctx* ctx, const std::vector<const xm_obj*>& recs,
                    xm_catalog* cat) {
     cat->ctx = ctx;
     cat->n = seg_table(recs, ns, hs, cat);
-    if (cat->seg.size() == 1) build_one(ctx, cat);
+    if (cat->n == 0) zoff_layout(ctx, cat);
+    else if (cat->seg.size() == 1) build_one(ctx, cat);
     else build_many(ctx, cat);
     finish_tables(ctx, cat);
 }

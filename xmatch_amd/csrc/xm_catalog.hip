@@ -3,17 +3,20 @@
 //
 // Replaces Sorter::Sort (BoostXmatch/BoostXmatch/Sorter.cu:141-185):
 //   GetZoneId + less_zonera (Sorter.cu:20-43)   -> k_make_keys (packed sort words)
-//   lower_bound/upper_bound (Sorter.cu:161-164) -> k_zone_offsets
 //   get_id / get_radec_radian (Sorter.cu:45-62) -> k_gather (one 32-byte row per source:
 //                                                  FP64 unit vector, radec2xyz of
 //                                                  Worker.cu:31-40 done once per source
 //                                                  instead of per pair, and the id)
+//   lower_bound/upper_bound (Sorter.cu:161-164) -> also k_gather: a row whose zone
+//                                                  differs from its left neighbour's
+//                                                  starts that zone (and every empty
+//                                                  zone between them) in zoff
 //
 // Many segments (the reference sorts each Segment on its own, one Sorter::Sort
 // per segment, BoostXmatch.cpp:250-263) are prepared together: k_seg_range finds
 // every segment's zone range, k_seg_words packs segment | zone | ra | row words in
-// a per-segment layout, one radix sort orders all of them, and k_gather_seg /
-// k_zone_offsets split them back into per-segment zone tables.
+// a per-segment layout, one radix sort orders all of them, and k_gather_seg
+// splits them back into per-segment zone tables.
 #include <algorithm>
 
 #include "xm_internal.h"
@@ -186,34 +189,54 @@ __global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ s
 }
 
 __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
-                                                    uint64_t* __restrict__ words, uint32_t n,
-                                                    Row64* __restrict__ rows) {
+                                                    const uint64_t* __restrict__ words, uint32_t n,
+                                                    Row64* __restrict__ rows,
+                                                    uint32_t* __restrict__ zoff, uint32_t n_zoff,
+                                                    uint64_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const uint64_t wd = words[i];
     const uint32_t s = seg_shift < 64 ? uint32_t(wd >> seg_shift) : 0u;
     const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
+    const uint64_t zmask = (uint64_t(1) << seg[s].zone_bits) - 1;
     const uint64_t loc = wd & ((uint64_t(1) << ib) - 1);
     const uint64_t ra_q = (wd >> ib) & ((uint64_t(1) << rb) - 1);
-    const uint64_t zrel = (wd >> (ib + rb)) & ((uint64_t(1) << seg[s].zone_bits) - 1);
+    const int32_t zrel = int32_t((wd >> (ib + rb)) & zmask);
+    // Zone offsets: rows of segment s hold zone-relative entries
+    // zoff[zoff_base + zrel]; the first row of a segment opens its zone 0.
+    int32_t zprev = -1;
+    if (i > 0) {
+        const uint64_t wp = words[i - 1];
+        const uint32_t sp = seg_shift < 64 ? uint32_t(wp >> seg_shift) : 0u;
+        if (sp == s) zprev = int32_t((wp >> (ib + rb)) & zmask);
+    }
+    const uint32_t base = seg[s].zoff_base;
+    for (int32_t z = zprev + 1; z <= zrel; z++) zoff[base + uint32_t(z)] = i;
+    if (i == n - 1) zoff[n_zoff - 1] = n;
     const xm_obj rec = seg[s].recs[loc];
     Row64 r;
     unit_vector(rec.ra_deg, rec.dec_deg, &r.x, &r.y, &r.z);
     r.id = rec.id;
     rows[i] = r;
-    const uint64_t zone = uint64_t(seg[s].zone_base) + zrel;
-    words[i] = (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb));
+    const uint64_t zone = uint64_t(seg[s].zone_base) + uint64_t(zrel);
+    keys[i] = (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb));
 }
 
 __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
-                                                uint64_t* __restrict__ words, uint32_t n,
-                                                PackLayout pl, Row64* __restrict__ rows) {
+                                                const uint64_t* __restrict__ words, uint32_t n,
+                                                PackLayout pl, int32_t zmin, Row64* __restrict__ rows,
+                                                uint32_t* __restrict__ zoff, uint32_t n_zoff,
+                                                uint64_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const uint64_t wd = words[i];
+    const int zs = pl.idx_bits + pl.ra_bits;
     const uint64_t s = wd & ((uint64_t(1) << pl.idx_bits) - 1);
     const uint64_t ra_q = (wd >> pl.idx_bits) & ((uint64_t(1) << pl.ra_bits) - 1);
-    const uint64_t zone = wd >> (pl.idx_bits + pl.ra_bits);
+    const int32_t zone = int32_t(wd >> zs);
+    const int32_t zprev = i > 0 ? int32_t(words[i - 1] >> zs) : zmin - 1;
+    for (int32_t z = zprev + 1; z <= zone; z++) zoff[uint32_t(z - zmin)] = i;
+    if (i == n - 1) zoff[n_zoff - 1] = n;
     const int64_t oid = recs[s].id;
     const double ra = recs[s].ra_deg;
     const double dec = recs[s].dec_deg;
@@ -221,32 +244,13 @@ __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
     unit_vector(ra, dec, &r.x, &r.y, &r.z);
     r.id = oid;
     rows[i] = r;
-    words[i] = (zone << RA_BITS) | (ra_q << (RA_BITS - pl.ra_bits));
+    keys[i] = (uint64_t(zone) << RA_BITS) | (ra_q << (RA_BITS - pl.ra_bits));
 }
 
 // zoff[zoff_base(s) + z - zmin(s)] = first row of segment s whose zone is >= z,
 // for z in [zmin(s), zmax(s)]; the entry after a segment's last is the next
 // segment's first row, and zoff[n_zoff - 1] = n.  Row i writes the entries of the
 // zones in (zone(i-1), zone(i)] of its segment (from zmin(s) at a segment start).
-__global__ __launch_bounds__(TPB) void k_zone_offsets(const uint64_t* __restrict__ keys, uint32_t n,
-                                                      const SegDesc* __restrict__ seg,
-                                                      uint32_t n_zoff, uint32_t* __restrict__ zoff) {
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
-    if (i > n) return;
-    if (i == n) {
-        zoff[n_zoff - 1] = n;
-        return;
-    }
-    const uint64_t k = keys[i];
-    const uint32_t s = key_seg(k);
-    const int32_t zmin = seg[s].zmin;
-    const uint32_t base = seg[s].zoff_base;
-    int32_t zp = zmin - 1;
-    if (i > 0 && key_seg(keys[i - 1]) == s) zp = key_zone(keys[i - 1]);
-    const int32_t zc = key_zone(k);
-    for (int32_t z = zp + 1; z <= zc; z++) zoff[base + uint32_t(z - zmin)] = i;
-}
-
 __global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict__ zoff,
                                                       uint32_t n_entries,
                                                       uint32_t* __restrict__ counts) {
@@ -297,12 +301,13 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
     });
 }
 
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n, PackLayout pl,
-                   Row64* rows) {
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
+                   PackLayout pl, int32_t zmin, Row64* rows, uint32_t* zoff, uint64_t n_zoff,
+                   uint64_t* keys) {
     if (n == 0) return;
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, words,
-                           uint32_t(n), pl, rows);
+                           uint32_t(n), pl, zmin, rows, zoff, uint32_t(n_zoff), keys);
     });
 }
 
@@ -323,20 +328,12 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
     });
 }
 
-void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, uint64_t* words, uint64_t n,
-                       Row64* rows) {
+void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
+                       uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys) {
     if (n == 0) return;
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather_seg, grid_for(n), dim3(TPB), 0, ctx->stream, seg, seg_shift,
-                           words, uint32_t(n), rows);
-    });
-}
-
-void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, const SegDesc* seg,
-                         uint64_t n_zoff, uint32_t* zoff) {
-    launch_timed(ctx, "zone_offsets", [&] {
-        hipLaunchKernelGGL(k_zone_offsets, grid_for(n + 1), dim3(TPB), 0, ctx->stream, keys,
-                           uint32_t(n), seg, uint32_t(n_zoff), zoff);
+                           words, uint32_t(n), rows, zoff, uint32_t(n_zoff), keys);
     });
 }
 

@@ -180,15 +180,17 @@ void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
                       uint64_t* words);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
-// carries) and, in place, words[i] -> catalog key.  The one-segment form takes
-// the layout by value; the many-segment form reads it from the segment table.
-void launch_gather(xm_ctx* ctx, const xm_obj* recs, uint64_t* words, uint64_t n, PackLayout pl,
-                   Row64* rows);
-void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, uint64_t* words, uint64_t n,
-                       Row64* rows);
-// zoff of every segment (see xm_catalog) from the sorted catalog keys.
-void launch_zone_offsets(xm_ctx* ctx, const uint64_t* keys, uint64_t n, const SegDesc* seg,
-                         uint64_t n_zoff, uint32_t* zoff);
+// carries), keys[i] (the catalog key) and every segment's zone offsets
+// zoff[zoff_base + z - zmin] (first row of zone z; the entry after a segment's
+// last zone is the next segment's first row, zoff[n_zoff - 1] = n).  words and
+// keys are distinct buffers (a row reads its left neighbour's word).  The
+// one-segment form takes the layout by value; the many-segment form reads it
+// from the segment table.
+void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
+                   PackLayout pl, int32_t zmin, Row64* rows, uint32_t* zoff, uint64_t n_zoff,
+                   uint64_t* keys);
+void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
+                       uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys);
 // counts[k] = chunks of zoff entry k (k < n_entries); chunks from the scanned counts.
 void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries, uint32_t* counts);
 void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
