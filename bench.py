@@ -34,12 +34,12 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 # algorithmic HBM bytes per source for the memory-bound kernels of a catalog build
 BYTES_PER_SOURCE = {
     "radix_scatter": 16.0,   # read packed word 8, write packed word 8
-    "radix_count": 8.0,      # read packed word
+    "radix_hist": 8.0,       # read packed word (once per sort, every digit)
     "make_keys": 32.0,       # read 24-B record, write packed word 8
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
 }
-KERNELS = ["make_keys", "radix_count", "radix_scatter", "scan_reduce", "scan_top",
-           "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
+KERNELS = ["make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
+           "scan_top", "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
            "pair_fp64", "region_scan", "compact"]
 
 
@@ -530,10 +530,10 @@ def main():
                 "algorithmic_bytes_per_launch": per_launch}
 
     sort_ms = sum(ktimes.get(k, {}).get("total_ms", 0.0)
-                  for k in ("radix_count", "radix_scatter", "scan_reduce", "scan_top",
-                            "scan_apply")) / args.steps
+                  for k in ("radix_hist", "radix_offsets", "radix_scatter")) / args.steps
     passes = ktimes.get("radix_scatter", {}).get("launches", 0) / max(1, args.steps) / 2
-    sort_bytes = 2 * args.n * passes * 24  # per pass: count 8, scatter 8 + 8; both catalogs
+    # per catalog: histogram read 8, then per pass scatter read 8 + write 8; both catalogs
+    sort_bytes = 2 * args.n * (8 + passes * 16)
 
     line = {
         "metric": "pair-tests/s", "value": value, "unit": "pair-tests/s", "n_gpus": 1,

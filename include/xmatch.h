@@ -95,13 +95,17 @@ xm_status xm_ctx_kernel_time(xm_ctx *ctx, const char *name, double *total_ms, ui
 xm_status xm_ctx_reset_timing(xm_ctx *ctx);
 /* Tuning knobs of the context (no counterpart in the reference, whose kernel
  * shape was fixed at compile time, Worker.cu:22,351-353):
- *   "dense_min"  a 64-row A chunk is joined densely (every B row of its window
- *                against every chunk row) when a B row is expected to meet at
- *                least this many chunk rows, else sparsely (each B row against the
- *                chunk rows within the ra window).  0 = always dense.  Default 16.
+ *   "dense_min"  an A chunk (up to 256 rows of one zone) is joined densely
+ *                (every B row of its window against every chunk row) when a B row
+ *                is expected to meet at least this many chunk rows, else sparsely
+ *                (each B row against the chunk rows within the ra window).
+ *                0 = always dense.  Default 16.
  *   "key_ra_bits" upper bound (1..24) on the ra bits kept in a catalog's sort key
  *                (applied at catalog creation; large catalogs keep fewer, so that
  *                zone | ra | row index fits 64 bits).  Default 24.
+ *   "sort_wide_words" 1 = the catalog sort's tile look-back uses 64-bit words at
+ *                every size (by default only from 2^30 rows up; 32-bit words
+ *                below).  Default 0.
  * Results never depend on the knobs; only the work done does. */
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);

@@ -71,6 +71,40 @@ def test_sorted_by_zone_then_ra(gpu_ctx):
     assert np.all(np.diff(ras)[same] >= -360.0 / 2**24)
 
 
+def test_sort_is_stable_and_word_width_invariant(gpu_ctx):
+    # Records that share (ra, dec) share their sort key: the radix sort must keep
+    # them in input order (ids rise with input position here).  ~70k rows make
+    # 18 sort tiles, so the tiles' look-back chains are exercised; 64-bit
+    # look-back words must give the same catalog as the default 32-bit words.
+    A, _, r = synth.config_small_patch(seed=5, n_a=60_000, n_b=10)
+    rng = np.random.default_rng(5)
+    dup = np.repeat(A[rng.choice(len(A), 2500, replace=False)], 4)
+    recs = np.concatenate([A, dup])
+    recs = recs[rng.permutation(len(recs))]
+    recs["id"] = np.arange(len(recs), dtype=np.int64) * 7 + 3
+    cat = gpu_ctx.catalog(recs, r)
+    e = cat.export()
+    cat.close()
+    pos = dict(zip(recs["id"].tolist(), range(len(recs))))
+    assert sorted(pos[i] for i in e["id"].tolist()) == list(range(len(recs)))
+    groups = {}
+    for i in e["id"].tolist():
+        groups.setdefault((recs["ra"][pos[i]], recs["dec"][pos[i]]), []).append(i)
+    assert sum(len(v) > 1 for v in groups.values()) == 2500
+    for k, ids in groups.items():
+        assert ids == sorted(ids), k
+    old = gpu_ctx.get_param("sort_wide_words")
+    try:
+        gpu_ctx.set_param("sort_wide_words", 1)
+        cat = gpu_ctx.catalog(recs, r)
+        e64 = cat.export()
+        cat.close()
+    finally:
+        gpu_ctx.set_param("sort_wide_words", old)
+    for f in e:
+        assert np.array_equal(e[f], e64[f]), f
+
+
 @pytest.mark.parametrize("r_arcsec", [1, 2, 5])
 def test_selfjoin_sample_data_golden(gpu_ctx, r_arcsec):
     a = synth.read_idradec(os.path.join(GOLD, "test.idradec"))

@@ -454,6 +454,7 @@ void catalog_build(xm_ctx* ctx, const std::vector<const xm_obj*>& recs,
     else if (cat->seg.size() == 1) build_one(ctx, cat);
     else build_many(ctx, cat);
     finish_tables(ctx, cat);
+    sort_check(ctx);
 }
 
 // A batch of jobs on the device: JobDesc table, per-job wave counts scanned into
@@ -599,6 +600,9 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "key_ra_bits") {
             if (value < 1 || value > RA_BITS) throw InvalidArg{"key_ra_bits must be in [1, 24]"};
             ctx->key_ra_bits = int(value);
+        } else if (n == "sort_wide_words") {
+            if (value != 0 && value != 1) throw InvalidArg{"sort_wide_words must be 0 or 1"};
+            ctx->sort_wide_words = value != 0;
         } else {
             throw InvalidArg{"unknown parameter '" + n + "'"};
         }
@@ -612,6 +616,7 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
         const std::string n = name;
         if (n == "dense_min") *value = ctx->dense_min;
         else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
+        else if (n == "sort_wide_words") *value = ctx->sort_wide_words ? 1 : 0;
         else throw InvalidArg{"unknown parameter '" + n + "'"};
     });
 }

@@ -51,6 +51,8 @@ struct xm_ctx {
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
     // pinned host scratch for small device->host reads
     void* pinned = nullptr;
+    // radix sort error word (device), set by a look-back that hit its polling limit
+    uint32_t* sort_err = nullptr;
     // device memory cache: blocks keyed by size class; reused in stream order
     std::vector<std::pair<size_t, void*>> free_blocks;          // (class bytes, ptr)
     std::vector<std::pair<void*, size_t>> live_blocks;          // ptr -> class bytes
@@ -58,6 +60,7 @@ struct xm_ctx {
     // tuning knobs (xm_ctx_set_param)
     uint32_t dense_min = 16;
     int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
+    bool sort_wide_words = false;  // 64-bit look-back words at every sort size
 };
 
 namespace xm {
@@ -166,6 +169,11 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
 // ping-ponged; on return *keys points at the sorted keys and *keys_alt at scratch.
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                      int hi_bit);
+// Reads the sort error word (synchronises the stream); throws if a sort since the
+// last check failed.
+void sort_check(xm_ctx* ctx);
+// byte offset of the sort error word's host copy in the context's pinned scratch
+constexpr size_t PIN_SORT_ERR_OFFSET = 1024;
 
 // ---- catalog kernels (xm_catalog.hip)
 // One-segment build: packed sort words (pack_word) of the records, their zone

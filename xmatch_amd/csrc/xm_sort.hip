@@ -5,14 +5,20 @@
 // on the zone and ra bits orders the segment by (zone, ra) without comparisons and
 // carries each row's input position in the low bits.
 //
-// Radix pass (8-bit digit), reduce-then-scan:
-//   k_radix_count   per 4096-key tile: digit histogram -> counts[digit][tile]
-//   scan            exclusive prefix over counts (digit-major) -> global offsets
-//   k_radix_scatter per tile: wave64 ballot match of equal digits gives each key
-//                   its stable rank; keys are regrouped by digit in LDS, then
-//                   written out so equal digits land in contiguous runs.
-// Algorithmic HBM traffic per pass and key: 8 (count) + 8 (scatter read) + 8
-// (scatter write) bytes.
+// Radix sort, one histogram read and then one kernel per 8-bit digit:
+//   k_radix_hist    one read of the keys: per-block histograms of every digit
+//   k_digit_offsets per digit position: global start of each digit value
+//   k_radix_scatter per 4096-key tile: wave64 ballot match of equal digits gives
+//                   each key its stable rank; the tile's offset for each digit is
+//                   the count of that digit in all earlier tiles, found by
+//                   decoupled look-back over the tiles' published counts; keys
+//                   are regrouped by digit in LDS, then written out so equal
+//                   digits land in contiguous runs.
+// Algorithmic HBM traffic per key: 8 bytes once (histogram), then 8 (read) + 8
+// (write) bytes per digit.
+#include <algorithm>
+#include <stdexcept>
+
 #include "xm_internal.h"
 
 namespace xm {
@@ -152,48 +158,119 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t* __restrict_
 }
 
 // ----------------------------------------------------------- radix sort ----
-__global__ __launch_bounds__(RS_THREADS) void k_radix_count(const uint64_t* __restrict__ keys,
-                                                            uint32_t n, int shift,
-                                                            uint32_t* __restrict__ counts,
-                                                            uint32_t nblocks) {
-    __shared__ uint32_t h[4][256];
-    const int w = threadIdx.x / WAVE;
-#pragma unroll
-    for (int q = 0; q < 4; q++) h[q][threadIdx.x] = 0;
+constexpr int MAX_PASSES = 8;          // 8-bit digits of a 64-bit key
+constexpr int HIST_ITEMS = 8;          // keys per thread per round of the histogram
+constexpr uint32_t HIST_BLOCKS = 512;  // histogram grid cap (partials to reduce)
+
+// Look-back words, one per (tile, digit value): top bit = pass parity, next bit
+// = inclusive prefix (count over this and all earlier tiles; else the tile's own
+// count), low bits = count.  Every tile writes all 256 of its words in every
+// pass, so after a pass all words carry that pass's parity and the next pass
+// tells its fresh words from stale ones by that bit; the buffer is cleared (parity
+// 0, which is not the first pass's) once per sort.  32-bit words hold counts below
+// 2^30, 64-bit words any count; the smaller words halve the look-back's traffic.
+template <typename W>
+struct LbWord {
+    static constexpr int TOP = int(sizeof(W) * 8) - 1;
+    static constexpr W COUNT_MASK = sizeof(W) == 4 ? W(0x3fffffffu) : W(0xffffffffu);
+    __device__ static W make(uint32_t parity, bool prefix, uint32_t count) {
+        return (W(parity & 1u) << TOP) | (W(prefix ? 1u : 0u) << (TOP - 1)) | W(count);
+    }
+    __device__ static bool fresh(W v, uint32_t parity) { return uint32_t(v >> TOP) == (parity & 1u); }
+    __device__ static bool prefix(W v) { return ((v >> (TOP - 1)) & 1u) != 0; }
+    __device__ static uint32_t count(W v) { return uint32_t(v & COUNT_MASK); }
+};
+// A look-back waits only on tiles that started before it and publish their own
+// counts before they wait on anything, so each wait ends.  The limit (seconds of
+// polling) turns a broken invariant into an error instead of a stalled kernel.
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
+// earlier tiles' words read per round trip of the look-back (measured on the
+// 50M-key sort: 4 beats 1, 2 and 8; wider windows cost more traffic than the
+// round trips they save)
+constexpr int LB_WINDOW = 4;
+
+// part[block][pass][digit]: counts of every digit value over the block's keys.
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
+                                                           uint32_t n, int lo_bit, int n_pass,
+                                                           uint32_t* __restrict__ part) {
+    __shared__ uint32_t h[MAX_PASSES * 256];
+    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS) h[i] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * RS_TILE;
+    const uint64_t stride = uint64_t(gridDim.x) * RS_THREADS * HIST_ITEMS;
+    for (uint64_t base = uint64_t(blockIdx.x) * RS_THREADS * HIST_ITEMS; base < n; base += stride) {
+        uint64_t k[HIST_ITEMS];
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
-        const uint32_t i = base + k * RS_THREADS + threadIdx.x;
-        if (i < n) {
-            const uint32_t d = uint32_t(keys[i] >> shift) & 255u;
-            atomicAdd(&h[w][d], 1u);
+        for (int j = 0; j < HIST_ITEMS; j++) {
+            const uint64_t i = base + uint64_t(j) * RS_THREADS + threadIdx.x;
+            k[j] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < HIST_ITEMS; j++) {
+            const uint64_t i = base + uint64_t(j) * RS_THREADS + threadIdx.x;
+            if (i < n) {
+                for (int p = 0; p < n_pass; p++)
+                    atomicAdd(&h[p * 256 + (uint32_t(k[j] >> (lo_bit + 8 * p)) & 255u)], 1u);
+            }
         }
     }
     __syncthreads();
-    const uint32_t t = threadIdx.x;
-    counts[size_t(t) * nblocks + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+    uint32_t* out = part + size_t(blockIdx.x) * n_pass * 256;
+    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS) out[i] = h[i];
 }
 
+// One block per digit position: start[pass][d] = keys whose digit is below d.
+// Thread t sums digit t & 255 over every fourth partial.
+constexpr int DO_THREADS = 1024;
+__global__ __launch_bounds__(DO_THREADS) void k_digit_offsets(const uint32_t* __restrict__ part,
+                                                              uint32_t n_part, int n_pass,
+                                                              uint32_t* __restrict__ start) {
+    __shared__ uint32_t s_sum[DO_THREADS];
+    __shared__ uint32_t s_w[DO_THREADS / WAVE + 1];
+    const int p = blockIdx.x;
+    const uint32_t d = threadIdx.x & 255u, q = threadIdx.x >> 8;
+    uint32_t sum = 0;
+#pragma unroll 16
+    for (uint32_t b = q; b < n_part; b += DO_THREADS / 256)
+        sum += part[(size_t(b) * n_pass + p) * 256 + d];
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    // threads 256.. add nothing, so the first 256 get the scan over the digits
+    const uint32_t t =
+        threadIdx.x < 256 ? s_sum[d] + s_sum[256 + d] + s_sum[512 + d] + s_sum[768 + d] : 0u;
+    const uint32_t ex = block_excl_scan<DO_THREADS>(t, s_w);
+    if (threadIdx.x < 256) start[p * 256 + d] = ex;
+}
+
+// One pass over the digit at `shift`.  Tiles take their index from tile_ctr in
+// the order they start, so every tile a look-back waits on is already running.
+// A tile publishes its digit counts (an LDS histogram taken as its keys arrive)
+// before it ranks them, and looks back only after ranking and regrouping, when
+// the earlier tiles have mostly published their prefixes.
+template <typename W>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, uint32_t n, int shift,
-    const uint32_t* __restrict__ offs, uint32_t nblocks) {
+    uint32_t parity, const uint32_t* __restrict__ digit_start, W* status, uint32_t* tile_ctr,
+    uint32_t* err) {
+    using LB = LbWord<W>;
     __shared__ uint64_t s_k[RS_TILE];
     __shared__ uint32_t s_wc[4][256];   // per-wave running digit counts, then wave prefixes
+    __shared__ uint32_t s_hist[256];    // the tile's digit counts
     __shared__ uint32_t s_ts[256];      // tile-local start of each digit
     __shared__ uint32_t s_go[256];      // global start of each digit for this tile
     __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
+    __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int w = tid / WAVE;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
 #pragma unroll
     for (int q = 0; q < 4; q++) s_wc[q][tid] = 0;
-    s_go[tid] = offs[size_t(tid) * nblocks + blockIdx.x];
+    s_hist[tid] = 0;
     __syncthreads();
+    const uint32_t tile = s_tile;
 
-    const uint64_t lt = (uint64_t(1) << lane) - 1u;
-    const uint32_t base = blockIdx.x * RS_TILE + w * RS_WAVE_TILE;
+    const uint32_t base = tile * RS_TILE + w * RS_WAVE_TILE;
     uint64_t key[RS_ITEMS];
     uint32_t rk[RS_ITEMS];
 #pragma unroll
@@ -201,6 +278,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         const uint32_t i = base + k * WAVE + lane;
         key[k] = i < n ? kin[i] : ~uint64_t(0);
     }
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const uint32_t i = base + k * WAVE + lane;
+        if (i < n) atomicAdd(&s_hist[uint32_t(key[k] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    const uint32_t cnt = s_hist[tid];
+    W* const mine = status + size_t(tile) * 256 + tid;
+    __hip_atomic_store(mine, LB::make(parity, tile == 0, cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+
+    const uint64_t lt = (uint64_t(1) << lane) - 1u;
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
@@ -223,15 +312,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     }
     __syncthreads();
     {
-        const uint32_t c0 = s_wc[0][tid], c1 = s_wc[1][tid], c2 = s_wc[2][tid], c3 = s_wc[3][tid];
+        const uint32_t c0 = s_wc[0][tid], c1 = s_wc[1][tid], c2 = s_wc[2][tid];
         s_wc[0][tid] = 0;
         s_wc[1][tid] = c0;
         s_wc[2][tid] = c0 + c1;
         s_wc[3][tid] = c0 + c1 + c2;
-        const uint32_t ex = block_excl_scan<RS_THREADS>(c0 + c1 + c2 + c3, s_w);
-        s_ts[tid] = ex;
     }
+    s_ts[tid] = block_excl_scan<RS_THREADS>(cnt, s_w);
     __syncthreads();
+    // regroup by digit in LDS (tile-local positions only)
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
@@ -240,13 +329,67 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             s_k[s_ts[d] + s_wc[w][d] + rk[k]] = key[k];
         }
     }
+    // look-back: sum the counts published by the earlier tiles, nearest first, up
+    // to and including the first inclusive prefix
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int32_t j = int32_t(tile) - 1;
+        uint32_t spins = 0;
+        for (;;) {
+            W v[LB_WINDOW];
+#pragma unroll
+            for (int k = 0; k < LB_WINDOW; k++) {
+                // tile 0 always holds a prefix, so the walk never passes it
+                v[k] = j - k >= 0 ? __hip_atomic_load(status + size_t(j - k) * 256 + tid,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : LB::make(parity, true, 0);
+            }
+            int used = 0;
+            bool done = false;
+#pragma unroll
+            for (int k = 0; k < LB_WINDOW; k++) {
+                if (!done && used == k && LB::fresh(v[k], parity)) {
+                    excl += LB::count(v[k]);
+                    used = k + 1;
+                    done = LB::prefix(v[k]);
+                }
+            }
+            if (done) break;
+            j -= used;
+            if (used == 0) {
+                if (++spins == LB_SPIN_LIMIT) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_go[tid] = digit_start[tid] + excl;
     __syncthreads();
-    const uint32_t tile0 = blockIdx.x * RS_TILE;
+    const uint32_t tile0 = tile * RS_TILE;
     const uint32_t tile_n = (n - tile0) < uint32_t(RS_TILE) ? (n - tile0) : uint32_t(RS_TILE);
     for (uint32_t p = tid; p < tile_n; p += RS_THREADS) {
         const uint64_t kk = s_k[p];
         const uint32_t d = uint32_t(kk >> shift) & 255u;
         kout[s_go[d] + (p - s_ts[d])] = kk;
+    }
+}
+
+template <typename W>
+void scatter_passes(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
+                    int n_pass, uint32_t nblocks, const uint32_t* start, void* status,
+                    uint32_t* ctr) {
+    for (int p = 0; p < n_pass; p++) {
+        launch_timed(ctx, "radix_scatter", [&] {
+            hipLaunchKernelGGL(k_radix_scatter<W>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
+                               (const uint64_t*)*keys, *keys_alt, uint32_t(n), lo_bit + 8 * p,
+                               uint32_t(p + 1), start + p * 256, static_cast<W*>(status),
+                               ctr + p, ctx->sort_err);
+        });
+        std::swap(*keys, *keys_alt);
     }
 }
 
@@ -281,27 +424,57 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
 
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                      int hi_bit) {
-    if (n <= 1) return;
+    if (n <= 1 || hi_bit <= lo_bit) return;
     if (n > 0xffffffffull) throw InvalidArg{"radix sort supports fewer than 2^32 keys"};
     if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
-    const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
-    uint32_t* counts = static_cast<uint32_t*>(dalloc(ctx, size_t(256) * nblocks * sizeof(uint32_t)));
     // 8-bit digits starting at lo_bit (the last one may reach past hi_bit into bits
     // that are equal in every key)
-    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
-        launch_timed(ctx, "radix_count", [&] {
-            hipLaunchKernelGGL(k_radix_count, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
-                               (const uint64_t*)*keys, uint32_t(n), shift, counts, nblocks);
-        });
-        scan_exclusive_u32(ctx, counts, uint64_t(256) * nblocks);
-        launch_timed(ctx, "radix_scatter", [&] {
-            hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
-                               (const uint64_t*)*keys, *keys_alt, uint32_t(n), shift,
-                               (const uint32_t*)counts, nblocks);
-        });
-        std::swap(*keys, *keys_alt);
+    const int n_pass = (hi_bit - lo_bit + 7) / 8;
+    const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
+    const uint32_t n_part = std::min<uint32_t>(
+        HIST_BLOCKS, uint32_t((n + RS_THREADS * HIST_ITEMS - 1) / (RS_THREADS * HIST_ITEMS)));
+    // a look-back that reaches its polling limit sets the context's sort error
+    // word (cleared once, reported by sort_check)
+    if (!ctx->sort_err) {
+        ctx->sort_err = static_cast<uint32_t*>(dalloc(ctx, sizeof(uint32_t)));
+        launch_fill_u32(ctx, ctx->sort_err, 1, 0u);
     }
-    dfree(ctx, counts);
+    // one block: look-back words | tile counters | partials | digit starts
+    // 32-bit look-back words suffice below 2^30 keys
+    const bool narrow = n < (uint64_t(1) << 30) && !ctx->sort_wide_words;
+    const size_t status_bytes = size_t(nblocks) * 256 * (narrow ? 4 : 8);
+    const size_t ctr_words = 64;   // MAX_PASSES tile counters, padded
+    const size_t part_words = size_t(n_part) * n_pass * 256;
+    char* blk = static_cast<char*>(
+        dalloc(ctx, status_bytes + (ctr_words + part_words + size_t(n_pass) * 256) * 4));
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(blk + status_bytes);
+    uint32_t* part = ctr + ctr_words;
+    uint32_t* start = part + part_words;
+    launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk), status_bytes / 4 + ctr_words, 0u);
+    launch_timed(ctx, "radix_hist", [&] {
+        hipLaunchKernelGGL(k_radix_hist, dim3(n_part), dim3(RS_THREADS), 0, ctx->stream,
+                           (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, part);
+    });
+    launch_timed(ctx, "radix_offsets", [&] {
+        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(DO_THREADS), 0, ctx->stream,
+                           (const uint32_t*)part, n_part, n_pass, start);
+    });
+    if (narrow)
+        scatter_passes<uint32_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
+    else
+        scatter_passes<uint64_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
+    dfree(ctx, blk);
+}
+
+void sort_check(xm_ctx* ctx) {
+    if (!ctx->sort_err) return;
+    uint32_t* pin = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->pinned) + PIN_SORT_ERR_OFFSET);
+    XM_HIP(hipMemcpyAsync(pin, ctx->sort_err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    XM_HIP(hipStreamSynchronize(ctx->stream));
+    if (*pin) {
+        launch_fill_u32(ctx, ctx->sort_err, 1, 0u);
+        throw std::runtime_error("radix sort: a tile's look-back did not complete");
+    }
 }
 
 }  // namespace xm
