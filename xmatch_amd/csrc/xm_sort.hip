@@ -6,10 +6,12 @@
 // carries each row's input position in the low bits.
 //
 // Radix sort, one histogram read and then one kernel per 8-bit digit:
-//   k_radix_hist    one read of the keys: per-block histograms of every digit
+//   k_radix_hist    one read of the keys: counts of every digit value at every
+//                   digit position (block histograms in LDS, added up atomically)
 //   k_digit_offsets per digit position: global start of each digit value
-//   k_radix_scatter per 4096-key tile: wave64 ballot match of equal digits gives
-//                   each key its stable rank; the tile's offset for each digit is
+//   k_radix_scatter per 4096-key tile: each key's stable rank among the equal
+//                   digits of its wavefront comes from a 64-bit lane mask per
+//                   digit built in LDS; the tile's offset for each digit is
 //                   the count of that digit in all earlier tiles, found by
 //                   decoupled look-back over the tiles' published counts; keys
 //                   are regrouped by digit in LDS, then written out so equal
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t* __restrict_
 // ----------------------------------------------------------- radix sort ----
 constexpr int MAX_PASSES = 8;          // 8-bit digits of a 64-bit key
 constexpr int HIST_ITEMS = 8;          // keys per thread per round of the histogram
-constexpr uint32_t HIST_BLOCKS = 512;  // histogram grid cap (partials to reduce)
+constexpr uint32_t HIST_BLOCKS = 1024; // histogram grid cap
 
 // Look-back words, one per (tile, digit value): top bit = pass parity, next bit
 // = inclusive prefix (count over this and all earlier tiles; else the tile's own
@@ -189,10 +191,11 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 // round trips they save)
 constexpr int LB_WINDOW = 4;
 
-// part[block][pass][digit]: counts of every digit value over the block's keys.
+// total[pass][digit] += counts of every digit value over the block's keys (total
+// starts at zero).
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
                                                            uint32_t n, int lo_bit, int n_pass,
-                                                           uint32_t* __restrict__ part) {
+                                                           uint32_t* __restrict__ total) {
     __shared__ uint32_t h[MAX_PASSES * 256];
     for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS) h[i] = 0;
     __syncthreads();
@@ -214,31 +217,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t* __res
         }
     }
     __syncthreads();
-    uint32_t* out = part + size_t(blockIdx.x) * n_pass * 256;
-    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS) out[i] = h[i];
+    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS)
+        if (h[i]) atomicAdd(&total[i], h[i]);
 }
 
 // One block per digit position: start[pass][d] = keys whose digit is below d.
-// Thread t sums digit t & 255 over every fourth partial.
-constexpr int DO_THREADS = 1024;
-__global__ __launch_bounds__(DO_THREADS) void k_digit_offsets(const uint32_t* __restrict__ part,
-                                                              uint32_t n_part, int n_pass,
+__global__ __launch_bounds__(RS_THREADS) void k_digit_offsets(const uint32_t* __restrict__ total,
                                                               uint32_t* __restrict__ start) {
-    __shared__ uint32_t s_sum[DO_THREADS];
-    __shared__ uint32_t s_w[DO_THREADS / WAVE + 1];
-    const int p = blockIdx.x;
-    const uint32_t d = threadIdx.x & 255u, q = threadIdx.x >> 8;
-    uint32_t sum = 0;
-#pragma unroll 16
-    for (uint32_t b = q; b < n_part; b += DO_THREADS / 256)
-        sum += part[(size_t(b) * n_pass + p) * 256 + d];
-    s_sum[threadIdx.x] = sum;
-    __syncthreads();
-    // threads 256.. add nothing, so the first 256 get the scan over the digits
-    const uint32_t t =
-        threadIdx.x < 256 ? s_sum[d] + s_sum[256 + d] + s_sum[512 + d] + s_sum[768 + d] : 0u;
-    const uint32_t ex = block_excl_scan<DO_THREADS>(t, s_w);
-    if (threadIdx.x < 256) start[p * 256 + d] = ex;
+    __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    start[i] = block_excl_scan<RS_THREADS>(total[i], s_w);
 }
 
 // One pass over the digit at `shift`.  Tiles take their index from tile_ctr in
@@ -256,7 +244,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __shared__ uint32_t s_wc[4][256];   // per-wave running digit counts, then wave prefixes
     __shared__ uint32_t s_hist[256];    // the tile's digit counts
     __shared__ uint32_t s_ts[256];      // tile-local start of each digit
-    __shared__ uint32_t s_go[256];      // global start of each digit for this tile
+    __shared__ uint32_t s_go[256];      // global start minus tile-local start of each digit
     __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
     __shared__ uint32_t s_tile;
 
@@ -267,6 +255,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 #pragma unroll
     for (int q = 0; q < 4; q++) s_wc[q][tid] = 0;
     s_hist[tid] = 0;
+    // per-wave digit masks for the ranking live in s_k until the regroup
+    uint64_t* const s_mask = s_k;
+#pragma unroll
+    for (int q = 0; q < 4; q++) s_mask[q * 256 + tid] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
 
@@ -295,20 +287,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         const uint32_t i = base + k * WAVE + lane;
         const bool valid = i < n;
         const uint32_t d = uint32_t(key[k] >> shift) & 255u;
-        // lanes holding the same digit: intersect the 8 per-bit ballots
-        uint64_t m = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            m &= bit ? bb : ~bb;
-        }
+        // lanes holding the same digit: each sets its bit in the digit's mask, then
+        // reads the whole mask back (a wavefront's LDS operations run in order)
+        uint64_t m = 0;
         uint32_t pre = 0;
-        if (valid) pre = s_wc[w][d];
+        if (valid) {
+            atomicOr(reinterpret_cast<unsigned long long*>(&s_mask[w * 256 + d]),
+                     (unsigned long long)(uint64_t(1) << lane));
+            m = s_mask[w * 256 + d];
+            pre = s_wc[w][d];
+        }
         rk[k] = pre + uint32_t(__popcll(m & lt));
-        // the highest lane of each digit group advances that digit's count; the
-        // wavefront's LDS read above is ordered before this write
-        if (valid && (m >> lane) == 1u) s_wc[w][d] = pre + uint32_t(__popcll(m));
+        // the highest lane of each digit group advances that digit's count and
+        // clears its mask, after the wavefront's reads above
+        if (valid && (m >> lane) == 1u) {
+            s_wc[w][d] = pre + uint32_t(__popcll(m));
+            s_mask[w * 256 + d] = 0;
+        }
     }
     __syncthreads();
     {
@@ -367,14 +362,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_go[tid] = digit_start[tid] + excl;
+    s_go[tid] = digit_start[tid] + excl - s_ts[tid];
     __syncthreads();
     const uint32_t tile0 = tile * RS_TILE;
     const uint32_t tile_n = (n - tile0) < uint32_t(RS_TILE) ? (n - tile0) : uint32_t(RS_TILE);
     for (uint32_t p = tid; p < tile_n; p += RS_THREADS) {
         const uint64_t kk = s_k[p];
-        const uint32_t d = uint32_t(kk >> shift) & 255u;
-        kout[s_go[d] + (p - s_ts[d])] = kk;
+        kout[s_go[uint32_t(kk >> shift) & 255u] + p] = kk;
     }
 }
 
@@ -431,7 +425,7 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     // that are equal in every key)
     const int n_pass = (hi_bit - lo_bit + 7) / 8;
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
-    const uint32_t n_part = std::min<uint32_t>(
+    const uint32_t hist_blocks = std::min<uint32_t>(
         HIST_BLOCKS, uint32_t((n + RS_THREADS * HIST_ITEMS - 1) / (RS_THREADS * HIST_ITEMS)));
     // a look-back that reaches its polling limit sets the context's sort error
     // word (cleared once, reported by sort_check)
@@ -439,25 +433,25 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
         ctx->sort_err = static_cast<uint32_t*>(dalloc(ctx, sizeof(uint32_t)));
         launch_fill_u32(ctx, ctx->sort_err, 1, 0u);
     }
-    // one block: look-back words | tile counters | partials | digit starts
+    // one block: look-back words | tile counters | digit totals | digit starts
     // 32-bit look-back words suffice below 2^30 keys
     const bool narrow = n < (uint64_t(1) << 30) && !ctx->sort_wide_words;
     const size_t status_bytes = size_t(nblocks) * 256 * (narrow ? 4 : 8);
     const size_t ctr_words = 64;   // MAX_PASSES tile counters, padded
-    const size_t part_words = size_t(n_part) * n_pass * 256;
-    char* blk = static_cast<char*>(
-        dalloc(ctx, status_bytes + (ctr_words + part_words + size_t(n_pass) * 256) * 4));
+    const size_t digit_words = size_t(n_pass) * 256;
+    char* blk = static_cast<char*>(dalloc(ctx, status_bytes + (ctr_words + 2 * digit_words) * 4));
     uint32_t* ctr = reinterpret_cast<uint32_t*>(blk + status_bytes);
-    uint32_t* part = ctr + ctr_words;
-    uint32_t* start = part + part_words;
-    launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk), status_bytes / 4 + ctr_words, 0u);
+    uint32_t* total = ctr + ctr_words;
+    uint32_t* start = total + digit_words;
+    launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk), status_bytes / 4 + ctr_words + digit_words,
+                    0u);
     launch_timed(ctx, "radix_hist", [&] {
-        hipLaunchKernelGGL(k_radix_hist, dim3(n_part), dim3(RS_THREADS), 0, ctx->stream,
-                           (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, part);
+        hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(RS_THREADS), 0, ctx->stream,
+                           (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, total);
     });
     launch_timed(ctx, "radix_offsets", [&] {
-        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(DO_THREADS), 0, ctx->stream,
-                           (const uint32_t*)part, n_part, n_pass, start);
+        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(RS_THREADS), 0, ctx->stream,
+                           (const uint32_t*)total, start);
     });
     if (narrow)
         scatter_passes<uint32_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
