@@ -9,7 +9,7 @@
 //   k_radix_hist    one read of the keys: counts of every digit value at every
 //                   digit position (block histograms in LDS, added up atomically)
 //   k_digit_offsets per digit position: global start of each digit value
-//   k_radix_scatter per 4096-key tile: each key's stable rank among the equal
+//   k_radix_scatter per 8192-key tile: each key's stable rank among the equal
 //                   digits of its wavefront comes from a 64-bit lane mask per
 //                   digit built in LDS; the tile's offset for each digit is
 //                   the count of that digit in all earlier tiles, found by
@@ -27,10 +27,15 @@ namespace xm {
 
 namespace {
 
-constexpr int RS_THREADS = 256;
+// 8192-key tiles of 8 wavefronts: measured on the 50M-key sort against 4096-key
+// (256 x 16, 256 x 12), 8192-key (256 x 32), 6144-, 12288- and 16384-key tiles.
+// Larger tiles mean fewer look-backs and longer runs of equal digits in the
+// write-out; at 512 x 16 two blocks still fit a CU (LDS) at 4 waves per SIMD.
+constexpr int RS_THREADS = 512;
 constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096 keys per tile
-constexpr int RS_WAVE_TILE = RS_TILE / 4;        // 1024 keys per wavefront
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // keys per tile
+constexpr int RS_WAVES = RS_THREADS / WAVE;
+constexpr int RS_WAVE_TILE = RS_TILE / RS_WAVES;  // keys per wavefront
 
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
@@ -161,6 +166,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t* __restrict_
 
 // ----------------------------------------------------------- radix sort ----
 constexpr int MAX_PASSES = 8;          // 8-bit digits of a 64-bit key
+constexpr int HIST_THREADS = 256;
 constexpr int HIST_ITEMS = 8;          // keys per thread per round of the histogram
 constexpr uint32_t HIST_BLOCKS = 1024; // histogram grid cap
 
@@ -193,23 +199,23 @@ constexpr int LB_WINDOW = 4;
 
 // total[pass][digit] += counts of every digit value over the block's keys (total
 // starts at zero).
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
                                                            uint32_t n, int lo_bit, int n_pass,
                                                            uint32_t* __restrict__ total) {
     __shared__ uint32_t h[MAX_PASSES * 256];
-    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS) h[i] = 0;
+    for (int i = threadIdx.x; i < n_pass * 256; i += HIST_THREADS) h[i] = 0;
     __syncthreads();
-    const uint64_t stride = uint64_t(gridDim.x) * RS_THREADS * HIST_ITEMS;
-    for (uint64_t base = uint64_t(blockIdx.x) * RS_THREADS * HIST_ITEMS; base < n; base += stride) {
+    const uint64_t stride = uint64_t(gridDim.x) * HIST_THREADS * HIST_ITEMS;
+    for (uint64_t base = uint64_t(blockIdx.x) * HIST_THREADS * HIST_ITEMS; base < n; base += stride) {
         uint64_t k[HIST_ITEMS];
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; j++) {
-            const uint64_t i = base + uint64_t(j) * RS_THREADS + threadIdx.x;
+            const uint64_t i = base + uint64_t(j) * HIST_THREADS + threadIdx.x;
             k[j] = i < n ? keys[i] : 0;
         }
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; j++) {
-            const uint64_t i = base + uint64_t(j) * RS_THREADS + threadIdx.x;
+            const uint64_t i = base + uint64_t(j) * HIST_THREADS + threadIdx.x;
             if (i < n) {
                 for (int p = 0; p < n_pass; p++)
                     atomicAdd(&h[p * 256 + (uint32_t(k[j] >> (lo_bit + 8 * p)) & 255u)], 1u);
@@ -217,16 +223,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint64_t* __res
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n_pass * 256; i += RS_THREADS)
+    for (int i = threadIdx.x; i < n_pass * 256; i += HIST_THREADS)
         if (h[i]) atomicAdd(&total[i], h[i]);
 }
 
 // One block per digit position: start[pass][d] = keys whose digit is below d.
-__global__ __launch_bounds__(RS_THREADS) void k_digit_offsets(const uint32_t* __restrict__ total,
-                                                              uint32_t* __restrict__ start) {
-    __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
+__global__ __launch_bounds__(256) void k_digit_offsets(const uint32_t* __restrict__ total,
+                                                       uint32_t* __restrict__ start) {
+    __shared__ uint32_t s_w[256 / WAVE + 1];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    start[i] = block_excl_scan<RS_THREADS>(total[i], s_w);
+    start[i] = block_excl_scan<256>(total[i], s_w);
 }
 
 // One pass over the digit at `shift`.  Tiles take their index from tile_ctr in
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     uint32_t* err) {
     using LB = LbWord<W>;
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_wc[4][256];   // per-wave running digit counts, then wave prefixes
+    __shared__ uint32_t s_wc[RS_WAVES][256];   // per-wave running digit counts, then prefixes
     __shared__ uint32_t s_hist[256];    // the tile's digit counts
     __shared__ uint32_t s_ts[256];      // tile-local start of each digit
     __shared__ uint32_t s_go[256];      // global start minus tile-local start of each digit
@@ -251,14 +257,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int w = tid / WAVE;
+    // threads below 256 own one digit value each in the per-digit steps
+    const bool dig = tid < 256;
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-#pragma unroll
-    for (int q = 0; q < 4; q++) s_wc[q][tid] = 0;
-    s_hist[tid] = 0;
     // per-wave digit masks for the ranking live in s_k until the regroup
     uint64_t* const s_mask = s_k;
-#pragma unroll
-    for (int q = 0; q < 4; q++) s_mask[q * 256 + tid] = 0;
+    static_assert(RS_TILE >= RS_WAVES * 256, "s_k holds the ranking masks");
+    for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) {
+        (&s_wc[0][0])[i] = 0;
+        s_mask[i] = 0;
+    }
+    if (dig) s_hist[tid] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
 
@@ -276,10 +285,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         if (i < n) atomicAdd(&s_hist[uint32_t(key[k] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    const uint32_t cnt = s_hist[tid];
+    const uint32_t cnt = dig ? s_hist[tid] : 0u;
     W* const mine = status + size_t(tile) * 256 + tid;
-    __hip_atomic_store(mine, LB::make(parity, tile == 0, cnt), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if (dig)
+        __hip_atomic_store(mine, LB::make(parity, tile == 0, cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 
     const uint64_t lt = (uint64_t(1) << lane) - 1u;
 #pragma unroll
@@ -306,14 +316,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         }
     }
     __syncthreads();
-    {
-        const uint32_t c0 = s_wc[0][tid], c1 = s_wc[1][tid], c2 = s_wc[2][tid];
-        s_wc[0][tid] = 0;
-        s_wc[1][tid] = c0;
-        s_wc[2][tid] = c0 + c1;
-        s_wc[3][tid] = c0 + c1 + c2;
+    if (dig) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; q++) {
+            const uint32_t c = s_wc[q][tid];
+            s_wc[q][tid] = run;
+            run += c;
+        }
     }
-    s_ts[tid] = block_excl_scan<RS_THREADS>(cnt, s_w);
+    {
+        const uint32_t ts = block_excl_scan<RS_THREADS>(cnt, s_w);
+        if (dig) s_ts[tid] = ts;
+    }
     __syncthreads();
     // regroup by digit in LDS (tile-local positions only)
 #pragma unroll
@@ -327,7 +342,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     // look-back: sum the counts published by the earlier tiles, nearest first, up
     // to and including the first inclusive prefix
     uint32_t excl = 0;
-    if (tile > 0) {
+    if (dig && tile > 0) {
         int32_t j = int32_t(tile) - 1;
         uint32_t spins = 0;
         for (;;) {
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_go[tid] = digit_start[tid] + excl - s_ts[tid];
+    if (dig) s_go[tid] = digit_start[tid] + excl - s_ts[tid];
     __syncthreads();
     const uint32_t tile0 = tile * RS_TILE;
     const uint32_t tile_n = (n - tile0) < uint32_t(RS_TILE) ? (n - tile0) : uint32_t(RS_TILE);
@@ -426,7 +441,7 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     const int n_pass = (hi_bit - lo_bit + 7) / 8;
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
     const uint32_t hist_blocks = std::min<uint32_t>(
-        HIST_BLOCKS, uint32_t((n + RS_THREADS * HIST_ITEMS - 1) / (RS_THREADS * HIST_ITEMS)));
+        HIST_BLOCKS, uint32_t((n + HIST_THREADS * HIST_ITEMS - 1) / (HIST_THREADS * HIST_ITEMS)));
     // a look-back that reaches its polling limit sets the context's sort error
     // word (cleared once, reported by sort_check)
     if (!ctx->sort_err) {
@@ -446,11 +461,11 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk), status_bytes / 4 + ctr_words + digit_words,
                     0u);
     launch_timed(ctx, "radix_hist", [&] {
-        hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(RS_THREADS), 0, ctx->stream,
+        hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(HIST_THREADS), 0, ctx->stream,
                            (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, total);
     });
     launch_timed(ctx, "radix_offsets", [&] {
-        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(RS_THREADS), 0, ctx->stream,
+        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(256), 0, ctx->stream,
                            (const uint32_t*)total, start);
     });
     if (narrow)
