@@ -345,8 +345,15 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     launch_fill_i32(ctx, small, 1, INT_MAX);
     launch_fill_i32(ctx, small + 1, 1, -1);
     launch_fill_i32(ctx, small + 2, 1, 0);
+    // the sort's digit counts, taken while the keys are written: digit positions
+    // from the row bits up over the ra bits and every zone bit (the sort may need
+    // fewer once the occupied zone range is known)
+    const int hist_pass = (pl.ra_bits + pl.zone_bits + 7) / 8;
+    uint32_t* digit_totals = static_cast<uint32_t*>(dalloc(ctx, size_t(hist_pass) * 256 * 4));
+    launch_fill_u32(ctx, digit_totals, uint64_t(hist_pass) * 256, 0u);
     launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
-                     reinterpret_cast<uint32_t*>(small + 2));
+                     reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
+                     digit_totals);
     int32_t* pin = static_cast<int32_t*>(ctx->pinned);
     XM_HIP(hipMemcpyAsync(pin, small, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
@@ -354,6 +361,7 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     const uint32_t nbad = uint32_t(pin[2]);
     dfree(ctx, small);
     if (nbad) {
+        dfree(ctx, digit_totals);
         dfree(ctx, words);
         dfree(ctx, words_alt);
         throw InvalidArg{std::to_string(nbad) +
@@ -367,7 +375,8 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     // order ties, which keep input order).
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
-    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit);
+    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit, digit_totals);
+    dfree(ctx, digit_totals);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     launch_gather(ctx, d.recs, words, n, pl, zmin, static_cast<Row64*>(cat->row), cat->zoff,

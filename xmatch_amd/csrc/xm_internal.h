@@ -167,8 +167,14 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
 // Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), in 8-bit digits
 // from lo_bit up; bits from hi_bit up must be equal in all keys.  Buffers are
 // ping-ponged; on return *keys points at the sorted keys and *keys_alt at scratch.
+// digit_totals, when given, holds the counts of every 8-bit digit value at digit
+// positions lo_bit + 8p (256 per position, at least the sort's passes), as
+// k_make_keys counts them while it writes the keys; else the sort reads the keys
+// once to count them.
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit);
+                     int hi_bit, const uint32_t* digit_totals = nullptr);
+// digit positions of a 64-bit key in 8-bit digits
+constexpr int SORT_MAX_PASSES = 8;
 // Reads the sort error word (synchronises the stream); throws if a sort since the
 // last check failed.
 void sort_check(xm_ctx* ctx);
@@ -178,8 +184,11 @@ constexpr size_t PIN_SORT_ERR_OFFSET = 1024;
 // ---- catalog kernels (xm_catalog.hip)
 // One-segment build: packed sort words (pack_word) of the records, their zone
 // range and bad count, in one pass.
+// With n_pass > 0 it also adds the counts of every digit value at digit positions
+// lo_bit + 8p (p < n_pass) of the valid records' words into digit_totals[p][256].
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
-                      PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad);
+                      PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad,
+                      int lo_bit = 0, int n_pass = 0, uint32_t* digit_totals = nullptr);
 // Many-segment build, pass 1: zone range and bad count of every segment
 // (range[3 s] = zmin, range[3 s + 1] = zmax, range[3 s + 2] = bad).
 void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range);

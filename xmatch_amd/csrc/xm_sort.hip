@@ -165,7 +165,6 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t* __restrict_
 }
 
 // ----------------------------------------------------------- radix sort ----
-constexpr int MAX_PASSES = 8;          // 8-bit digits of a 64-bit key
 constexpr int HIST_THREADS = 256;
 constexpr int HIST_ITEMS = 8;          // keys per thread per round of the histogram
 constexpr uint32_t HIST_BLOCKS = 1024; // histogram grid cap
@@ -202,7 +201,7 @@ constexpr int LB_WINDOW = 4;
 __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
                                                            uint32_t n, int lo_bit, int n_pass,
                                                            uint32_t* __restrict__ total) {
-    __shared__ uint32_t h[MAX_PASSES * 256];
+    __shared__ uint32_t h[SORT_MAX_PASSES * 256];
     for (int i = threadIdx.x; i < n_pass * 256; i += HIST_THREADS) h[i] = 0;
     __syncthreads();
     const uint64_t stride = uint64_t(gridDim.x) * HIST_THREADS * HIST_ITEMS;
@@ -432,7 +431,7 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
 }
 
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit) {
+                     int hi_bit, const uint32_t* digit_totals) {
     if (n <= 1 || hi_bit <= lo_bit) return;
     if (n > 0xffffffffull) throw InvalidArg{"radix sort supports fewer than 2^32 keys"};
     if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
@@ -452,21 +451,23 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     // 32-bit look-back words suffice below 2^30 keys
     const bool narrow = n < (uint64_t(1) << 30) && !ctx->sort_wide_words;
     const size_t status_bytes = size_t(nblocks) * 256 * (narrow ? 4 : 8);
-    const size_t ctr_words = 64;   // MAX_PASSES tile counters, padded
+    const size_t ctr_words = 64;   // SORT_MAX_PASSES tile counters, padded
     const size_t digit_words = size_t(n_pass) * 256;
     char* blk = static_cast<char*>(dalloc(ctx, status_bytes + (ctr_words + 2 * digit_words) * 4));
     uint32_t* ctr = reinterpret_cast<uint32_t*>(blk + status_bytes);
     uint32_t* total = ctr + ctr_words;
     uint32_t* start = total + digit_words;
-    launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk), status_bytes / 4 + ctr_words + digit_words,
-                    0u);
-    launch_timed(ctx, "radix_hist", [&] {
-        hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(HIST_THREADS), 0, ctx->stream,
-                           (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, total);
-    });
+    launch_fill_u32(ctx, reinterpret_cast<uint32_t*>(blk),
+                    status_bytes / 4 + ctr_words + (digit_totals ? 0 : digit_words), 0u);
+    if (!digit_totals) {
+        launch_timed(ctx, "radix_hist", [&] {
+            hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(HIST_THREADS), 0, ctx->stream,
+                               (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, total);
+        });
+    }
     launch_timed(ctx, "radix_offsets", [&] {
         hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(256), 0, ctx->stream,
-                           (const uint32_t*)total, start);
+                           digit_totals ? digit_totals : (const uint32_t*)total, start);
     });
     if (narrow)
         scatter_passes<uint32_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
