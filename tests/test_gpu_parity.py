@@ -105,6 +105,29 @@ def test_sort_is_stable_and_word_width_invariant(gpu_ctx):
         assert np.array_equal(e[f], e64[f]), f
 
 
+@pytest.mark.parametrize("order", ["dec", "dec_ra", "reversed"])
+def test_ordered_input_same_set(gpu_ctx, order):
+    # Spatially ordered input (as survey catalogs are often stored) gives whole
+    # wavefronts of equal zone digits in the sort; the join must not change.
+    A, B, r = synth.config_small_patch(seed=6, n_a=80_000, n_b=80_000)
+    if order == "dec":
+        A = A[np.argsort(A["dec"], kind="stable")]
+        B = B[np.argsort(B["dec"], kind="stable")]
+    elif order == "dec_ra":
+        A = A[np.lexsort((A["ra"], A["dec"]))]
+        B = B[np.lexsort((B["ra"], B["dec"]))]
+    else:
+        A = A[np.lexsort((A["ra"], A["dec"]))][::-1].copy()
+        B = B[np.lexsort((B["ra"], B["dec"]))][::-1].copy()
+    ref, _ = O.match(A, B, r)
+    got, _ = gpu_join(gpu_ctx, A, B, r)
+    assert_same(got, ref)
+    cat = gpu_ctx.catalog(A, r)
+    e = cat.export()
+    cat.close()
+    assert np.all(np.diff(e["zone"]) >= 0)
+
+
 @pytest.mark.parametrize("r_arcsec", [1, 2, 5])
 def test_selfjoin_sample_data_golden(gpu_ctx, r_arcsec):
     a = synth.read_idradec(os.path.join(GOLD, "test.idradec"))

@@ -175,6 +175,7 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
                      int hi_bit, const uint32_t* digit_totals = nullptr);
 // digit positions of a 64-bit key in 8-bit digits
 constexpr int SORT_MAX_PASSES = 8;
+
 // Reads the sort error word (synchronises the stream); throws if a sort since the
 // last check failed.
 void sort_check(xm_ctx* ctx);

@@ -196,6 +196,24 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 // round trips they save)
 constexpr int LB_WINDOW = 4;
 
+// Adds one key's digits at positions lo_bit + 8p (p < n_pass) to the LDS
+// histogram h[p][256]; the active lanes of a wavefront call it together.  A digit
+// that all 64 lanes hold (segment and zone bits of ordered input, the constant top
+// digits) is added once, by lane 0: 64 atomics on one LDS word would run one after
+// another.
+__device__ inline void count_digits(uint32_t* h, uint64_t key, bool valid, int lo_bit,
+                                    int n_pass) {
+    const bool full = __ballot(valid) == ~uint64_t(0);
+    for (int p = 0; p < n_pass; p++) {
+        const uint32_t d = uint32_t(key >> (lo_bit + 8 * p)) & 255u;
+        if (full && __all(d == uint32_t(__builtin_amdgcn_readfirstlane(int(d))))) {
+            if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&h[p * 256 + d], uint32_t(WAVE));
+        } else if (valid) {
+            atomicAdd(&h[p * 256 + d], 1u);
+        }
+    }
+}
+
 // total[pass][digit] += counts of every digit value over the block's keys (total
 // starts at zero).
 __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
@@ -215,10 +233,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __r
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; j++) {
             const uint64_t i = base + uint64_t(j) * HIST_THREADS + threadIdx.x;
-            if (i < n) {
-                for (int p = 0; p < n_pass; p++)
-                    atomicAdd(&h[p * 256 + (uint32_t(k[j] >> (lo_bit + 8 * p)) & 255u)], 1u);
-            }
+            count_digits(h, k[j], i < n, lo_bit, n_pass);
         }
     }
     __syncthreads();
@@ -296,6 +311,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         const uint32_t i = base + k * WAVE + lane;
         const bool valid = i < n;
         const uint32_t d = uint32_t(key[k] >> shift) & 255u;
+        // Valid lanes are a prefix of the wavefront (keys run from the tile start),
+        // so lane 0 holds a valid digit whenever any lane does.
+        const uint32_t d0 = uint32_t(__builtin_amdgcn_readfirstlane(int(d)));
+        if (__all(!valid || d == d0)) {
+            // one digit across the wavefront: the valid lanes are its group
+            const uint64_t m = __ballot(valid);
+            const uint32_t pre = s_wc[w][d0];
+            rk[k] = pre + uint32_t(__popcll(m & lt));
+            if (valid && (m >> lane) == 1u) s_wc[w][d0] = pre + uint32_t(__popcll(m));
+            continue;
+        }
         // lanes holding the same digit: each sets its bit in the digit's mask, then
         // reads the whole mask back (a wavefront's LDS operations run in order)
         uint64_t m = 0;
