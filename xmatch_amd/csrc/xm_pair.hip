@@ -286,9 +286,7 @@ __device__ inline void stage_hits(uint64_t m, double d2, uint32_t ra, uint32_t r
 // same B zones, share an L2, and every XCD gets runs from the whole range (chunks
 // are in zone order, and a zone's cost follows its population, which varies with
 // declination: contiguous eighths of the range would leave the polar XCDs idle).
-#ifndef XCD_RUN
-#define XCD_RUN 16
-#endif
+constexpr uint32_t XCD_RUN = 16;
 __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
     constexpr uint32_t X = 8, G = XCD_RUN;
     const uint32_t full = nb / (X * G) * (X * G);
