@@ -186,10 +186,12 @@ def run_batch(args):
     seg_b = xm.SegmentTable(b_views, rs)
     jobs = xm.job_table([(j, j, rs[j]) for j in range(n_jobs)])
 
+    prec = xm.FP32_FILTER_FP64_RECHECK if args.precision == "fp32" else xm.FP64_PLAIN
+
     def batched():
         A = ctx.catalog_segments(seg_a)
         B = ctx.catalog_segments(seg_b)
-        res = ctx.match_batch(A, B, jobs)
+        res = ctx.match_batch(A, B, jobs, precision=prec)
         return A, B, res
 
     ctx.set_timing(True)
@@ -206,7 +208,7 @@ def run_batch(args):
         if i < args.steps - 1:
             res.close(); A.close(); B.close()
     kernels = {}
-    for k in KERNELS + ["seg_range", "job_prep"]:
+    for k in KERNELS + ["seg_range", "job_prep", "pair_fp32", "to_row32"]:
         tot, cnt = ctx.kernel_time(k)
         if cnt:
             kernels[k] = {"total_ms": tot / args.steps, "launches": cnt // args.steps}
@@ -232,7 +234,7 @@ def run_batch(args):
         for j in range(n_jobs):
             ca = ctx.catalog(a_views[j], rs[j])
             cb = ctx.catalog(b_views[j], rs[j])
-            r = ctx.match(ca, cb, rs[j])
+            r = ctx.match(ca, cb, rs[j], precision=prec)
             counts[j] = r.count
             r.close(); cb.close(); ca.close()
         return counts
@@ -248,7 +250,8 @@ def run_batch(args):
     sample = sorted(set(rng.choice(n_jobs, size=min(24, n_jobs), replace=False).tolist()))
     sa = ctx.catalog_segments([a_views[j] for j in sample], [rs[j] for j in sample])
     sb = ctx.catalog_segments([b_views[j] for j in sample], [rs[j] for j in sample])
-    sres = ctx.match_batch(sa, sb, [(i, i, rs[j]) for i, j in enumerate(sample)])
+    sres = ctx.match_batch(sa, sb, [(i, i, rs[j]) for i, j in enumerate(sample)],
+                           precision=prec)
     per_job = sres.by_job()
     sample_ok = True
     for i, j in enumerate(sample):
@@ -268,7 +271,8 @@ def run_batch(args):
                                "1e6], cap radius U[0.1, 2] deg, r ~ logU[0.5, 10] arcsec, h = r"
                                % n_jobs,
                    "n_jobs": n_jobs, "sources": n_src, "b": "50% copies (0.3 arcsec) + 50% fresh",
-                   "precision": "fp64_plain"},
+                   "precision": ("fp32_filter_fp64_recheck" if args.precision == "fp32"
+                                 else "fp64_plain")},
         "steps": args.steps, "warmup": args.warmup,
         "batched_ms": t_batch, "loop_ms": t_loop, "batched_over_loop": t_loop / t_batch,
         "value": (w["w_cand"] / (t_batch / 1e3)) if w["w_cand"] else None,
@@ -426,6 +430,8 @@ def main():
     ap.add_argument("--duration", type=float, default=10.0,
                     help="serve config: seconds of arrivals per rate")
     ap.add_argument("--jobs", type=int, default=10_000, help="jobs of the batch config")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="batch config: FP64 plain or the FP32 filter with FP64 recheck")
     ap.add_argument("--no-work-counts", action="store_true",
                     help="batch config: skip the host work counts")
     ap.add_argument("--gpus", type=int, default=1)
