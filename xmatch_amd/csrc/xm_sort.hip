@@ -20,6 +20,7 @@
 // (write) bytes per digit.
 #include <algorithm>
 #include <stdexcept>
+#include <string>
 
 #include "xm_internal.h"
 
@@ -459,7 +460,9 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                      int hi_bit, const uint32_t* digit_totals) {
     if (n <= 1 || hi_bit <= lo_bit) return;
-    if (n > 0xffffffffull) throw InvalidArg{"radix sort supports fewer than 2^32 keys"};
+    // tile indices are 32-bit: the last tile's end must not pass 2^32
+    if (n > 0xffffffffull - RS_TILE)
+        throw InvalidArg{"radix sort supports up to 2^32 - " + std::to_string(RS_TILE + 1) + " keys"};
     if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
     // 8-bit digits starting at lo_bit (the last one may reach past hi_bit into bits
     // that are equal in every key)
