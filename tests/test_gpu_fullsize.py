@@ -1,0 +1,108 @@
+"""The full-sky 50M x 50M join at 2 arcsec (the benchmark workload) checked through
+properties that do not depend on size, since the oracle cannot join it whole in a
+test's time:
+- the catalog is ordered by (zone, ra key) and its unit vectors equal the oracle's
+  bit for bit on a sample of rows;
+- joining B against A gives exactly the swapped pairs of A against B;
+- every pair is within the radius by the oracle's chord test, and its distance
+  agrees with one recomputed on the host;
+- in a band of 1000 zones at the equator (0.49% of A) the pairs equal the oracle's
+  join of that band."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xmatch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+N = 50_000_000
+DIST_TOL_ARCSEC = 1e-6
+
+
+@pytest.fixture(scope="module")
+def full(gpu_ctx):
+    A, B, r = synth.config_full_sky(seed=2, n_a=N, n_b=N)
+    dA = gpu_ctx.to_device(A)
+    dB = gpu_ctx.to_device(B)
+    ca = gpu_ctx.catalog(dA, r)
+    cb = gpu_ctx.catalog(dB, r)
+    res = gpu_ctx.match(ca, cb, r)
+    ab = res.to_host()
+    res.close()
+    res = gpu_ctx.match(cb, ca, r)
+    ba = res.to_host()
+    res.close()
+    ea = ca.export()
+    info = ca.info()
+    ca.close()
+    cb.close()
+    del dA, dB
+    gpu_ctx.release_memory()
+    return A, B, r, ab, ba, ea, info
+
+
+def _pair_key(ia, ib):
+    # A ids are 0..N-1 (< 2^26), B ids 1e9 + 0..N-1 (< 2^31): one sortable int64
+    return ia.astype(np.int64) * (1 << 31) + ib.astype(np.int64)
+
+
+def test_full_catalog_order_and_unit_vectors(full):
+    A, _, r, _, _, e, info = full
+    assert np.array_equal(np.sort(e["id"]), np.arange(N, dtype=np.int64))
+    assert np.all(np.diff(e["zone"]) >= 0)
+    assert info["zmin"] == e["zone"][0] and info["zmax"] == e["zone"][-1]
+    # within a zone, ra ascends up to the key's ra granularity (19 ra bits at 50M rows)
+    ra = A["ra"][e["id"]]
+    same = np.diff(e["zone"]) == 0
+    assert np.all(np.diff(ra)[same] >= -360.0 / 2**19)
+    rng = np.random.default_rng(11)
+    s = rng.choice(N, 1_000_000, replace=False)
+    x, y, z = O.unitvec(A["ra"][e["id"][s]], A["dec"][e["id"][s]])
+    assert np.array_equal(e["x"][s].view(np.int64), x.view(np.int64))
+    assert np.array_equal(e["y"][s].view(np.int64), y.view(np.int64))
+    assert np.array_equal(e["z"][s].view(np.int64), z.view(np.int64))
+
+
+def test_full_swap_symmetry(full):
+    _, _, _, ab, ba, _, _ = full
+    assert len(ab[0]) == len(ba[0]) > 29_000_000
+    k1 = _pair_key(ab[0], ab[1])
+    k2 = _pair_key(ba[1], ba[0])
+    o1 = np.argsort(k1, kind="stable")
+    o2 = np.argsort(k2, kind="stable")
+    assert np.array_equal(k1[o1], k2[o2])
+    assert np.abs(ab[2][o1] - ba[2][o2]).max() <= DIST_TOL_ARCSEC
+
+
+def test_full_pairs_within_radius(full):
+    A, B, r, ab, _, _, _ = full
+    ia, ib, d = ab
+    xa, ya, za = O.unitvec(A["ra"][ia], A["dec"][ia])
+    jb = ib - synth.B_ID_OFFSET
+    xb, yb, zb = O.unitvec(B["ra"][jb], B["dec"][jb])
+    dx, dy, dz = xa - xb, ya - yb, za - zb
+    d2 = dx * dx + dy * dy + dz * dz
+    assert np.all(d2 < O.sr_dist2(r))
+    d_host = np.degrees(2.0 * np.arcsin(np.sqrt(d2) / 2.0)) * 3600.0
+    assert np.abs(d_host - d).max() <= DIST_TOL_ARCSEC
+
+
+def test_full_equator_band_vs_oracle(full):
+    A, B, r, ab, _, _, _ = full
+    nzn = int(np.ceil(180.0 / r))
+    z_lo, z_hi = nzn // 2 - 500, nzn // 2 + 499
+    dec_lo, dec_hi = z_lo * r - 90.0, (z_hi + 1) * r - 90.0
+    cand = A[(A["dec"] >= dec_lo - r) & (A["dec"] <= dec_hi + r)]
+    zc = O.zone(cand["dec"], r)
+    a_band = cand[(zc >= z_lo) & (zc <= z_hi)]
+    b_band = B[(B["dec"] >= dec_lo - 3 * r) & (B["dec"] <= dec_hi + 3 * r)]
+    assert 200_000 < len(a_band) < 300_000   # 0.49% of the sphere
+    ref, _ = O.match(a_band, b_band, r, threads=16)
+    sel = np.isin(ab[0], a_band["id"])
+    k_gpu = _pair_key(ab[0][sel], ab[1][sel])
+    o = np.argsort(k_gpu, kind="stable")
+    k_ref = _pair_key(ref[0], ref[1])
+    assert len(k_ref) > 100_000
+    assert np.array_equal(k_gpu[o], k_ref)
+    assert np.abs(ab[2][sel][o] - ref[2]).max() <= DIST_TOL_ARCSEC
