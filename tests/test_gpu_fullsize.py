@@ -106,3 +106,34 @@ def test_full_equator_band_vs_oracle(full):
     assert len(k_ref) > 100_000
     assert np.array_equal(k_gpu[o], k_ref)
     assert np.abs(ab[2][sel][o] - ref[2]).max() <= DIST_TOL_ARCSEC
+
+
+def _digest(ida, idb, dist):
+    # order-independent: sums of 64-bit mixes of the pairs and of the distance bits
+    h = (ida.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ \
+        (idb.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F))
+    h ^= h >> np.uint64(29)
+    return (len(ida), int(np.sum(h, dtype=np.uint64)),
+            int(np.sum(dist.view(np.uint64), dtype=np.uint64)))
+
+
+def test_full_plane_fp32_filter_equals_fp64(gpu_ctx):
+    # The 200M x 200M galactic-plane join at 1 arcsec (records generated on the
+    # device): the FP32 filter with FP64 recheck returns the FP64 set, distances
+    # bit for bit.
+    import xmatch_amd as xm
+    n, r = 200_000_000, 1.0 / 3600.0
+    plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
+    dA = gpu_ctx.synth_fresh(plane, n, 2003, stream=0, id0=0)
+    dB = gpu_ctx.synth_copies(dA, 0.5, 0.3, plane, n, 2003, stream=1)
+    ca = gpu_ctx.catalog(dA, r)
+    cb = gpu_ctx.catalog(dB, r)
+    dig = {}
+    for name, prec in (("fp64", xm.FP64_PLAIN), ("fp32", xm.FP32_FILTER_FP64_RECHECK)):
+        res = gpu_ctx.match(ca, cb, r, precision=prec)
+        dig[name] = _digest(*res.to_host(sort=False))
+        res.close()
+    ca.close(); cb.close(); dA.free(); dB.free()
+    gpu_ctx.release_memory()
+    assert 99_500_000 < dig["fp64"][0] < 101_000_000
+    assert dig["fp32"] == dig["fp64"]
