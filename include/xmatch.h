@@ -106,7 +106,11 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  *   "sort_wide_words" 1 = the catalog sort's tile look-back uses 64-bit words at
  *                every size (by default only from 2^30 rows up; 32-bit words
  *                below).  Default 0.
- * Results never depend on the knobs; only the work done does. */
+ *   "sort_fail_test" 1 = every catalog sort of more than one tile (8192 rows)
+ *                fails through its look-back's bounded wait, and catalog creation
+ *                returns XM_EINTERNAL: for tests of that error path.  Default 0.
+ * Results never depend on the knobs (but "sort_fail_test"); only the work done
+ * does. */
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
 /* Free and total device memory; blocks the context caches for reuse count as free. */

@@ -105,6 +105,31 @@ def test_sort_is_stable_and_word_width_invariant(gpu_ctx):
         assert np.array_equal(e[f], e64[f]), f
 
 
+def test_sort_lookback_failure_is_reported(gpu_ctx):
+    # With "sort_fail_test" the sort's tiles publish no counts and poll only 64
+    # times, so every look-back ends at its limit (the kernels still exit): the
+    # catalog call must fail with XM_EINTERNAL, and the context must then sort
+    # correctly again.
+    A, _, r = synth.config_small_patch(seed=7, n_a=60_000, n_b=10)   # 8 tiles
+    assert gpu_ctx.get_param("sort_fail_test") == 0
+    try:
+        gpu_ctx.set_param("sort_fail_test", 1)
+        with pytest.raises(xm.XmatchError) as ei:
+            gpu_ctx.catalog(A, r)
+        assert ei.value.status == xm.XM_EINTERNAL
+        assert "look-back" in str(ei.value)
+    finally:
+        gpu_ctx.set_param("sort_fail_test", 0)
+    cat = gpu_ctx.catalog(A, r)
+    e = cat.export()
+    cat.close()
+    assert np.array_equal(np.sort(e["id"]), np.sort(A["id"]))
+    assert np.all(np.diff(e["zone"]) >= 0)
+    ref, _ = O.match(A, A, r)
+    got, _ = gpu_join(gpu_ctx, A, A, r)
+    assert_same(got, ref)
+
+
 @pytest.mark.parametrize("order", ["dec", "dec_ra", "reversed"])
 def test_ordered_input_same_set(gpu_ctx, order):
     # Spatially ordered input (as survey catalogs are often stored) gives whole

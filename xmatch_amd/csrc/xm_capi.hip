@@ -612,6 +612,9 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "sort_wide_words") {
             if (value != 0 && value != 1) throw InvalidArg{"sort_wide_words must be 0 or 1"};
             ctx->sort_wide_words = value != 0;
+        } else if (n == "sort_fail_test") {
+            if (value != 0 && value != 1) throw InvalidArg{"sort_fail_test must be 0 or 1"};
+            ctx->sort_fail_test = value != 0;
         } else {
             throw InvalidArg{"unknown parameter '" + n + "'"};
         }
@@ -626,6 +629,7 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
         if (n == "dense_min") *value = ctx->dense_min;
         else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
         else if (n == "sort_wide_words") *value = ctx->sort_wide_words ? 1 : 0;
+        else if (n == "sort_fail_test") *value = ctx->sort_fail_test ? 1 : 0;
         else throw InvalidArg{"unknown parameter '" + n + "'"};
     });
 }

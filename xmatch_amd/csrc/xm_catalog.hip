@@ -214,9 +214,10 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
                                                     const uint64_t* __restrict__ words, uint32_t n,
                                                     Row64* __restrict__ rows,
                                                     uint32_t* __restrict__ zoff, uint32_t n_zoff,
-                                                    uint64_t* __restrict__ keys) {
+                                                    uint64_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ sort_err) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || *sort_err) return;
     const uint64_t wd = words[i];
     const uint32_t s = seg_shift < 64 ? uint32_t(wd >> seg_shift) : 0u;
     const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
@@ -244,13 +245,17 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
     keys[i] = (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb));
 }
 
+// A sort whose look-back failed (sort_err set) leaves words that may index past
+// the records and the zone table: the gathers then write nothing (the catalog call
+// reports the error).
 __global__ __launch_bounds__(TPB) void k_gather(const xm_obj* __restrict__ recs,
                                                 const uint64_t* __restrict__ words, uint32_t n,
                                                 PackLayout pl, int32_t zmin, Row64* __restrict__ rows,
                                                 uint32_t* __restrict__ zoff, uint32_t n_zoff,
-                                                uint64_t* __restrict__ keys) {
+                                                uint64_t* __restrict__ keys,
+                                                const uint32_t* __restrict__ sort_err) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || *sort_err) return;
     const uint64_t wd = words[i];
     const int zs = pl.idx_bits + pl.ra_bits;
     const uint64_t s = wd & ((uint64_t(1) << pl.idx_bits) - 1);
@@ -330,9 +335,10 @@ void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint6
                    PackLayout pl, int32_t zmin, Row64* rows, uint32_t* zoff, uint64_t n_zoff,
                    uint64_t* keys) {
     if (n == 0) return;
+    const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather, grid_for(n), dim3(TPB), 0, ctx->stream, recs, words,
-                           uint32_t(n), pl, zmin, rows, zoff, uint32_t(n_zoff), keys);
+                           uint32_t(n), pl, zmin, rows, zoff, uint32_t(n_zoff), keys, err);
     });
 }
 
@@ -356,9 +362,10 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
                        uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys) {
     if (n == 0) return;
+    const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather_seg, grid_for(n), dim3(TPB), 0, ctx->stream, seg, seg_shift,
-                           words, uint32_t(n), rows, zoff, uint32_t(n_zoff), keys);
+                           words, uint32_t(n), rows, zoff, uint32_t(n_zoff), keys, err);
     });
 }
 

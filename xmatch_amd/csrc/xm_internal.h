@@ -61,6 +61,7 @@ struct xm_ctx {
     uint32_t dense_min = 16;
     int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
     bool sort_wide_words = false;  // 64-bit look-back words at every sort size
+    bool sort_fail_test = false;   // sorts fail by their look-back limit (error-path tests)
 };
 
 namespace xm {
@@ -176,6 +177,10 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
 // digit positions of a 64-bit key in 8-bit digits
 constexpr int SORT_MAX_PASSES = 8;
 
+// The context's sort error word (device, allocated and cleared on first use): set
+// by a look-back that reached its polling limit; later passes and the gathers do
+// nothing while it is set.
+uint32_t* sort_error_word(xm_ctx* ctx);
 // Reads the sort error word (synchronises the stream); throws if a sort since the
 // last check failed.
 void sort_check(xm_ctx* ctx);

@@ -259,7 +259,7 @@ template <typename W>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, uint32_t n, int shift,
     uint32_t parity, const uint32_t* __restrict__ digit_start, W* status, uint32_t* tile_ctr,
-    uint32_t* err) {
+    uint32_t* err, uint32_t spin_limit, uint32_t publish) {
     using LB = LbWord<W>;
     __shared__ uint64_t s_k[RS_TILE];
     __shared__ uint32_t s_wc[RS_WAVES][256];   // per-wave running digit counts, then prefixes
@@ -268,13 +268,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __shared__ uint32_t s_go[256];      // global start minus tile-local start of each digit
     __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_abort;
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int w = tid / WAVE;
     // threads below 256 own one digit value each in the per-digit steps
     const bool dig = tid < 256;
-    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    if (tid == 0) {
+        // a failed look-back in an earlier pass left holes of stale words in this
+        // pass's input; their digits could place keys past n, so the pass stops
+        s_abort = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_tile = s_abort ? 0u : atomicAdd(tile_ctr, 1u);
+    }
     // per-wave digit masks for the ranking live in s_k until the regroup
     uint64_t* const s_mask = s_k;
     static_assert(RS_TILE >= RS_WAVES * 256, "s_k holds the ranking masks");
@@ -284,6 +290,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     }
     if (dig) s_hist[tid] = 0;
     __syncthreads();
+    if (s_abort) return;  // the whole block: s_abort is one LDS word
     const uint32_t tile = s_tile;
 
     const uint32_t base = tile * RS_TILE + w * RS_WAVE_TILE;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __syncthreads();
     const uint32_t cnt = dig ? s_hist[tid] : 0u;
     W* const mine = status + size_t(tile) * 256 + tid;
-    if (dig)
+    if (dig && publish)
         __hip_atomic_store(mine, LB::make(parity, tile == 0, cnt), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 
@@ -393,15 +400,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             if (done) break;
             j -= used;
             if (used == 0) {
-                if (++spins == LB_SPIN_LIMIT) {
+                if (++spins >= spin_limit) {
                     atomicOr(err, 1u);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        if (publish)
+            __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
     if (dig) s_go[tid] = digit_start[tid] + excl - s_ts[tid];
     __syncthreads();
@@ -417,12 +425,19 @@ template <typename W>
 void scatter_passes(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                     int n_pass, uint32_t nblocks, const uint32_t* start, void* status,
                     uint32_t* ctr) {
+    // a look-back that reaches its polling limit sets the context's sort error
+    // word (reported by sort_check)
+    uint32_t* const err = sort_error_word(ctx);
     for (int p = 0; p < n_pass; p++) {
         launch_timed(ctx, "radix_scatter", [&] {
             hipLaunchKernelGGL(k_radix_scatter<W>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
                                (const uint64_t*)*keys, *keys_alt, uint32_t(n), lo_bit + 8 * p,
                                uint32_t(p + 1), start + p * 256, static_cast<W*>(status),
-                               ctr + p, ctx->sort_err);
+                               ctr + p, err,
+                               // a context set to fail its sorts (tests of the error
+                               // path): tiles publish nothing and poll 64 times
+                               ctx->sort_fail_test ? 64u : LB_SPIN_LIMIT,
+                               ctx->sort_fail_test ? 0u : 1u);
         });
         std::swap(*keys, *keys_alt);
     }
@@ -457,6 +472,14 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
     dfree(ctx, partial);
 }
 
+uint32_t* sort_error_word(xm_ctx* ctx) {
+    if (!ctx->sort_err) {
+        ctx->sort_err = static_cast<uint32_t*>(dalloc(ctx, sizeof(uint32_t)));
+        launch_fill_u32(ctx, ctx->sort_err, 1, 0u);
+    }
+    return ctx->sort_err;
+}
+
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                      int hi_bit, const uint32_t* digit_totals) {
     if (n <= 1 || hi_bit <= lo_bit) return;
@@ -470,12 +493,6 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
     const uint32_t hist_blocks = std::min<uint32_t>(
         HIST_BLOCKS, uint32_t((n + HIST_THREADS * HIST_ITEMS - 1) / (HIST_THREADS * HIST_ITEMS)));
-    // a look-back that reaches its polling limit sets the context's sort error
-    // word (cleared once, reported by sort_check)
-    if (!ctx->sort_err) {
-        ctx->sort_err = static_cast<uint32_t*>(dalloc(ctx, sizeof(uint32_t)));
-        launch_fill_u32(ctx, ctx->sort_err, 1, 0u);
-    }
     // one block: look-back words | tile counters | digit totals | digit starts
     // 32-bit look-back words suffice below 2^30 keys
     const bool narrow = n < (uint64_t(1) << 30) && !ctx->sort_wide_words;
