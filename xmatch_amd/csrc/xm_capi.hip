@@ -444,8 +444,13 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
                           ctx->stream));
     uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
     uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-    launch_seg_words(ctx, cat->seg_dev, S, n, seg_shift, words);
-    radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits);
+    // the sort's digit counts, taken while the words are written
+    const int n_pass = (content + seg_bits - idx_bits + 7) / 8;
+    uint32_t* digit_totals = static_cast<uint32_t*>(dalloc(ctx, size_t(n_pass) * 256 * 4));
+    launch_fill_u32(ctx, digit_totals, uint64_t(n_pass) * 256, 0u);
+    launch_seg_words(ctx, cat->seg_dev, S, n, seg_shift, words, idx_bits, n_pass, digit_totals);
+    radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits, digit_totals);
+    dfree(ctx, digit_totals);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row),
                       cat->zoff, cat->n_zoff, words_alt);

@@ -108,9 +108,9 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
             atomicAdd(&digit_totals[i], t);
 }
 
-// ---- many segments: a block takes a tile of SEG_TILE consecutive rows of the
-// concatenation and walks the segments that overlap it, so every segment-wide
-// quantity costs one atomic per (block, segment).
+// ---- many segments: a block takes a run of consecutive rows of the concatenation
+// (k_seg_range: a tile of SEG_TILE rows) and walks the segments that overlap it, so
+// every segment-wide quantity costs one atomic per (block, segment).
 constexpr uint32_t SEG_TILE = TPB * 16;
 
 // Last segment whose row_base <= row (segments are in row order; empty segments
@@ -185,29 +185,46 @@ __global__ __launch_bounds__(TPB) void k_seg_range(const SegDesc* __restrict__ s
     }
 }
 
+// Each block takes one contiguous run of rows (finding its first segment once) and
+// walks the segments in it.  Like k_make_keys, it also counts the digits of its
+// words for the radix sort (segment bits are uniform over long runs, so
+// count_digits adds a wavefront's shared digit once) and adds its counts to
+// digit_totals at the end.
 __global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ seg, uint32_t n_seg,
                                                    uint64_t n, int seg_shift,
-                                                   uint64_t* __restrict__ words) {
-    const uint64_t t0 = uint64_t(blockIdx.x) * SEG_TILE;
-    const uint64_t t1 = t0 + SEG_TILE < n ? t0 + SEG_TILE : n;
-    for (uint32_t s = seg_of_row(seg, n_seg, t0); s < n_seg; s++) {
-        const uint64_t base = seg[s].row_base, end = base + seg[s].n;
-        if (base >= t1) break;
-        if (end <= t0) continue;
-        const uint64_t lo = base > t0 ? base : t0, hi = end < t1 ? end : t1;
-        const xm_obj* __restrict__ recs = seg[s].recs;
-        const double h = seg[s].h_deg;
-        const int32_t nzs = seg[s].n_zones, zb = seg[s].zone_base;
-        const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
-        const uint64_t top = seg_shift < 64 ? uint64_t(s) << seg_shift : 0;
-        for (uint64_t i = lo + threadIdx.x; i < hi; i += TPB) {
-            const double ra = recs[i - base].ra_deg;
-            const double dec = recs[i - base].dec_deg;
-            const int32_t z = zone_of(dec, h, nzs) - zb;
-            const uint64_t q = ra_quantum(ra) >> (RA_BITS - rb);
-            words[i] = top | (uint64_t(uint32_t(z)) << (rb + ib)) | (q << ib) | (i - base);
+                                                   uint64_t* __restrict__ words, int lo_bit,
+                                                   int n_pass, uint32_t* __restrict__ digit_totals) {
+    __shared__ uint32_t s_h[SORT_MAX_PASSES * 256];
+    for (int i = threadIdx.x; i < n_pass * 256; i += TPB) s_h[i] = 0;
+    __syncthreads();
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t t0 = uint64_t(blockIdx.x) * per;
+    const uint64_t t1 = t0 + per < n ? t0 + per : n;
+    if (t0 < t1) {
+        for (uint32_t s = seg_of_row(seg, n_seg, t0); s < n_seg; s++) {
+            const uint64_t base = seg[s].row_base, end = base + seg[s].n;
+            if (base >= t1) break;
+            if (end <= t0) continue;
+            const uint64_t lo = base > t0 ? base : t0, hi = end < t1 ? end : t1;
+            const xm_obj* __restrict__ recs = seg[s].recs;
+            const double h = seg[s].h_deg;
+            const int32_t nzs = seg[s].n_zones, zb = seg[s].zone_base;
+            const int ib = seg[s].idx_bits, rb = seg[s].ra_bits;
+            const uint64_t top = seg_shift < 64 ? uint64_t(s) << seg_shift : 0;
+            for (uint64_t i = lo + threadIdx.x; i < hi; i += TPB) {
+                const double ra = recs[i - base].ra_deg;
+                const double dec = recs[i - base].dec_deg;
+                const int32_t z = zone_of(dec, h, nzs) - zb;
+                const uint64_t q = ra_quantum(ra) >> (RA_BITS - rb);
+                const uint64_t word = top | (uint64_t(uint32_t(z)) << (rb + ib)) | (q << ib) | (i - base);
+                words[i] = word;
+                count_digits(s_h, word, true, lo_bit, n_pass);
+            }
         }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_pass * 256; i += TPB)
+        if (s_h[i]) atomicAdd(&digit_totals[i], s_h[i]);
 }
 
 __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
@@ -351,11 +368,14 @@ void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 }
 
 void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
-                      uint64_t* words) {
+                      uint64_t* words, int lo_bit, int n_pass, uint32_t* digit_totals) {
     if (n == 0 || n_seg == 0) return;
+    if (n_pass < 0 || n_pass > SORT_MAX_PASSES || (n_pass > 0 && !digit_totals))
+        throw InvalidArg{"seg_words: bad digit histogram request"};
+    const uint32_t nblk = uint32_t(std::min<uint64_t>((n + TPB - 1) / TPB, MAKE_KEYS_BLOCKS));
     launch_timed(ctx, "make_keys", [&] {
-        hipLaunchKernelGGL(k_seg_words, dim3(uint32_t((n + SEG_TILE - 1) / SEG_TILE)), dim3(TPB), 0,
-                           ctx->stream, seg, n_seg, n, seg_shift, words);
+        hipLaunchKernelGGL(k_seg_words, dim3(nblk), dim3(TPB), 0, ctx->stream, seg, n_seg, n,
+                           seg_shift, words, lo_bit, n_pass, digit_totals);
     });
 }
 

@@ -177,6 +177,24 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
 // digit positions of a 64-bit key in 8-bit digits
 constexpr int SORT_MAX_PASSES = 8;
 
+// Adds one key's digits at positions lo_bit + 8p (p < n_pass) to the LDS
+// histogram h[p][256]; the active lanes of a wavefront call it together.  A digit
+// that all 64 lanes hold (segment and zone bits of ordered input, the constant top
+// digits) is added once, by lane 0: 64 atomics on one LDS word would run one after
+// another.
+__device__ inline void count_digits(uint32_t* h, uint64_t key, bool valid, int lo_bit,
+                                    int n_pass) {
+    const bool full = __ballot(valid) == ~uint64_t(0);
+    for (int p = 0; p < n_pass; p++) {
+        const uint32_t d = uint32_t(key >> (lo_bit + 8 * p)) & 255u;
+        if (full && __all(d == uint32_t(__builtin_amdgcn_readfirstlane(int(d))))) {
+            if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&h[p * 256 + d], uint32_t(WAVE));
+        } else if (valid) {
+            atomicAdd(&h[p * 256 + d], 1u);
+        }
+    }
+}
+
 // The context's sort error word (device, allocated and cleared on first use): set
 // by a look-back that reached its polling limit; later passes and the gathers do
 // nothing while it is set.
@@ -201,7 +219,8 @@ void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 // Pass 2: words = s << seg_shift | (zone - zone_base) | ra | row within the segment,
 // in the segment's layout.
 void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
-                      uint64_t* words);
+                      uint64_t* words, int lo_bit = 0, int n_pass = 0,
+                      uint32_t* digit_totals = nullptr);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
 // carries), keys[i] (the catalog key) and every segment's zone offsets
 // zoff[zoff_base + z - zmin] (first row of zone z; the entry after a segment's

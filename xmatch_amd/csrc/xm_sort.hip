@@ -197,24 +197,6 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 // round trips they save)
 constexpr int LB_WINDOW = 4;
 
-// Adds one key's digits at positions lo_bit + 8p (p < n_pass) to the LDS
-// histogram h[p][256]; the active lanes of a wavefront call it together.  A digit
-// that all 64 lanes hold (segment and zone bits of ordered input, the constant top
-// digits) is added once, by lane 0: 64 atomics on one LDS word would run one after
-// another.
-__device__ inline void count_digits(uint32_t* h, uint64_t key, bool valid, int lo_bit,
-                                    int n_pass) {
-    const bool full = __ballot(valid) == ~uint64_t(0);
-    for (int p = 0; p < n_pass; p++) {
-        const uint32_t d = uint32_t(key >> (lo_bit + 8 * p)) & 255u;
-        if (full && __all(d == uint32_t(__builtin_amdgcn_readfirstlane(int(d))))) {
-            if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&h[p * 256 + d], uint32_t(WAVE));
-        } else if (valid) {
-            atomicAdd(&h[p * 256 + d], 1u);
-        }
-    }
-}
-
 // total[pass][digit] += counts of every digit value over the block's keys (total
 // starts at zero).
 __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
