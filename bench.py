@@ -443,7 +443,7 @@ def main():
                     help="full_sky config: record order of both catalogs (dec = sorted by "
                          "declination; a diagnostic form, not the headline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-dec-band", type=float, default=6.0)
+    ap.add_argument("--cpu-dec-band", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (for scaling PMC byte counters)")
