@@ -355,57 +355,81 @@ struct SparseCtx {
     const int32_t* s_aq;      // ra quanta of the chunk rows, padded with INT_MAX
 };
 
-// One B row per lane (row j, key); tested against the chunk rows within +-aq of
-// its ra if it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).
-// The chunk's ra quanta are in LDS; a candidate's A unit vector is read from
-// memory (the chunk's rows are contiguous, so these reads share cache lines).
+// Candidate pairs of the sparse path wait in a per-wavefront LDS ring until a
+// full round of 64 can be tested, one pair per lane: the B-row scan finds about
+// one candidate per row, so testing in the scan loop left most lanes idle.
+constexpr uint32_t CQ = 128;  // ring entries (a round is drained once 64 wait)
+struct CandQ {
+    uint32_t* a;              // A row of each queued pair
+    uint32_t* b;              // B row
+    uint32_t head, tail;      // wave-uniform
+};
+
 template <bool F32>
-__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key, int ni,
-                                  const int32_t* lq, const int32_t* hq, const int32_t* shift,
+__device__ inline void cand_round(CandQ& q, uint32_t k,
                                   const typename Prec<F32>::Row* __restrict__ arow,
                                   const typename Prec<F32>::Row* __restrict__ brow,
                                   typename Prec<F32>::T thr, uint32_t lane, Stage& st,
                                   const Out& o) {
     using T = typename Prec<F32>::T;
-    const int32_t q = int32_t(key & RA_MASK);
+    wave_lds_sync();  // the ring entries written by other lanes
+    T d2 = 0;
+    bool hit = false;
+    uint32_t ra = 0, rb = 0;
+    if (lane < k) {
+        const uint32_t s = (q.head + lane) & (CQ - 1);
+        ra = q.a[s];
+        rb = q.b[s];
+        const auto a = arow[ra];
+        const auto b = brow[rb];
+        d2 = chord2<T>(a.x, a.y, a.z, b.x, b.y, b.z);
+        hit = d2 < thr;
+    }
+    q.head += k;
+    const uint64_t m = __ballot(hit);
+    if (m) stage_hits<F32>(m, double(d2), ra, rb, lane, st, o);
+}
+
+// One B row per lane (row j, key), against the chunk rows within +-aq of its ra if
+// it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).  The chunk's
+// ra quanta are in LDS: a binary search finds the first, a scan the rest, and each
+// candidate (A row, B row) goes to the ring; full rounds are tested at once.
+template <bool F32>
+__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key,
+                                  int ni, const int32_t* lq, const int32_t* hq,
+                                  const int32_t* shift, CandQ& q,
+                                  const typename Prec<F32>::Row* __restrict__ arow,
+                                  const typename Prec<F32>::Row* __restrict__ brow,
+                                  typename Prec<F32>::T thr, uint32_t lane, Stage& st,
+                                  const Out& o) {
+    const int32_t qv = int32_t(key & RA_MASK);
     bool in = false;
     int32_t qb = 0;
     if (valid) {
         for (int t = 0; t < ni; t++)
-            if (q >= lq[t] && q <= hq[t]) {
+            if (qv >= lq[t] && qv <= hq[t]) {
                 in = true;
-                qb = q + shift[t];
+                qb = qv + shift[t];
             }
     }
     if (!__ballot(in)) return;
     const int32_t lo_q = qb - x.aq, hi_q = qb + x.aq;
-    // c = number of chunk rows with quantum < lo_q (branchless binary search)
     uint32_t c = 0;
     for (uint32_t step = x.half; step >= 1; step >>= 1)
         if (x.s_aq[c + step - 1] < lo_q) c += step;
     if (x.s_aq[c] < lo_q) c++;
-    bool have_b = false;
-    T bx = 0, by = 0, bz = 0;
     for (uint32_t k = 0;; k++) {
         const uint32_t cc = c + k;
         const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(CHUNK_ROWS) ? cc : 0] <= hi_q;
-        if (!__ballot(act)) break;
-        T d2 = 0;
-        bool hit = false;
+        const uint64_t m = __ballot(act);
+        if (!m) break;
         if (act) {
-            if (!have_b) {
-                const auto b = brow[j];
-                bx = b.x;
-                by = b.y;
-                bz = b.z;
-                have_b = true;
-            }
-            const auto a = arow[x.start + cc];
-            d2 = chord2<T>(a.x, a.y, a.z, bx, by, bz);
-            hit = d2 < thr;
+            const uint32_t s = (q.tail + mbcnt64(m)) & (CQ - 1);
+            q.a[s] = x.start + cc;
+            q.b[s] = j;
         }
-        const uint64_t m = __ballot(hit);
-        if (m) stage_hits<F32>(m, double(d2), x.start + cc, j, lane, st, o);
+        q.tail += uint32_t(__popcll(m));
+        if (q.tail - q.head >= uint32_t(WAVE)) cand_round<F32>(q, WAVE, arow, brow, thr, lane, st, o);
     }
 }
 
@@ -464,6 +488,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
     __shared__ int32_t s_aq[PK_WAVES][CHUNK_ROWS];
+    __shared__ uint32_t s_qa[PK_WAVES][CQ];
+    __shared__ uint32_t s_qb[PK_WAVES][CQ];
 
     const uint32_t lane = lane_id();
     const uint32_t w = uni(threadIdx.x / WAVE);
@@ -540,6 +566,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     uint32_t half = 1;
     while (2 * half < cnt) half *= 2;
     const SparseCtx x{start, cnt, half, int32_t(aq), s_aq[w]};
+    CandQ q{s_qa[w], s_qb[w], 0, 0};
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
         const int32_t zb = za + dz;
         if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
@@ -567,7 +594,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 if (b0 + k * WAVE >= b1) break;
                 const uint32_t j = b0 + k * WAVE + lane;
-                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, arow_t, brow_t, thr,
+                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, q, arow_t, brow_t, thr,
                                 lane, st, o);
             }
         } else {
@@ -578,11 +605,15 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
                     const uint32_t j = jb + lane;
                     const bool valid = j < j1;
                     const uint64_t key = valid ? bkey[j] : 0;
-                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], arow_t,
+                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], q, arow_t,
                                     brow_t, thr, lane, st, o);
                 }
             }
         }
+    }
+    while (q.tail != q.head) {
+        const uint32_t k = q.tail - q.head < uint32_t(WAVE) ? q.tail - q.head : uint32_t(WAVE);
+        cand_round<F32>(q, k, arow_t, brow_t, thr, lane, st, o);
     }
     if (st.n) flush_stage<F32>(st, lane, o);
 }
