@@ -38,7 +38,7 @@ BYTES_PER_SOURCE = {
     "make_keys": 32.0,       # read 24-B record, write packed word 8
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
 }
-KERNELS = ["make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
+KERNELS = ["layout", "make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
            "scan_top", "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
            "pair_fp64", "region_scan", "compact"]
 

@@ -52,10 +52,11 @@ def test_full_catalog_order_and_unit_vectors(full):
     assert np.array_equal(np.sort(e["id"]), np.arange(N, dtype=np.int64))
     assert np.all(np.diff(e["zone"]) >= 0)
     assert info["zmin"] == e["zone"][0] and info["zmax"] == e["zone"][-1]
-    # within a zone, ra ascends up to the key's ra granularity (19 ra bits at 50M rows)
+    # within a zone, ra ascends up to the key's ra granularity (at least 13 ra bits are
+    # kept at 50M rows: 6 of 19 are dropped when that saves a sort pass)
     ra = A["ra"][e["id"]]
     same = np.diff(e["zone"]) == 0
-    assert np.all(np.diff(ra)[same] >= -360.0 / 2**19)
+    assert np.all(np.diff(ra)[same] >= -360.0 / 2**13)
     rng = np.random.default_rng(11)
     s = rng.choice(N, 1_000_000, replace=False)
     x, y, z = O.unitvec(A["ra"][e["id"][s]], A["dec"][e["id"][s]])

@@ -318,25 +318,7 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     if (n == 0) return;
     PackLayout pl = pack_layout(n, d.n_zones);
     if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
-    // The sort runs over the zone and ra bits in 8-bit digits.  When they are a
-    // few bits over a whole number of digits, dropping those ra bits saves a pass.
-    // The coarser ra widens each B row's search by the granularity: it is done
-    // when that adds under 0.01 rows of a zone (at the catalog's mean zone
-    // population) to each search, about a tenth of the candidates a search finds
-    // at zone height = radius (measured: 0.04 rows per search doubled the tests
-    // of the 50M x 50M full-sky join and cost what the saved pass gained).
-    {
-        const int total = pl.ra_bits + pl.zone_bits;
-        const int fewer = total / 8 * 8 - pl.zone_bits;
-        if (total % 8 != 0 && fewer >= 1 && fewer < pl.ra_bits) {
-            const double rows_per_zone = double(n) / double(d.n_zones);
-            const double gran = std::ldexp(1.0, RA_BITS - fewer);
-            if (rows_per_zone * 2.0 * gran / std::ldexp(1.0, RA_BITS) <= 0.01) pl.ra_bits = fewer;
-        }
-    }
-    d.ra_gran = int64_t(1) << (RA_BITS - pl.ra_bits);
     d.idx_bits = pl.idx_bits;
-    d.ra_bits = pl.ra_bits;
     d.zone_bits = pl.zone_bits;
     d.zone_base = 0;
     uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
@@ -345,6 +327,10 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     launch_fill_i32(ctx, small, 1, INT_MAX);
     launch_fill_i32(ctx, small + 1, 1, -1);
     launch_fill_i32(ctx, small + 2, 1, 0);
+    // the ra bits the sort and the key drop, decided on the device from a sample
+    uint32_t* stats = static_cast<uint32_t*>(dalloc(ctx, LAYOUT_STATS_WORDS * sizeof(uint32_t)));
+    launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, stats,
+                  reinterpret_cast<uint32_t*>(small + 3));
     // the sort's digit counts, taken while the keys are written: digit positions
     // from the row bits up over the ra bits and every zone bit (the sort may need
     // fewer once the occupied zone range is known)
@@ -353,13 +339,17 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     launch_fill_u32(ctx, digit_totals, uint64_t(hist_pass) * 256, 0u);
     launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
                      reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
-                     digit_totals);
+                     digit_totals, reinterpret_cast<uint32_t*>(small + 3));
     int32_t* pin = static_cast<int32_t*>(ctx->pinned);
-    XM_HIP(hipMemcpyAsync(pin, small, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    XM_HIP(hipMemcpyAsync(pin, small, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
     const int32_t zmin = pin[0], zmax = pin[1];
     const uint32_t nbad = uint32_t(pin[2]);
+    const int ra_drop = pin[3];
     dfree(ctx, small);
+    dfree(ctx, stats);
+    d.ra_bits = pl.ra_bits - ra_drop;
+    d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     if (nbad) {
         dfree(ctx, digit_totals);
         dfree(ctx, words);
@@ -375,12 +365,12 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     // order ties, which keep input order).
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
-    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits, hi_bit, digit_totals);
+    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals);
     dfree(ctx, digit_totals);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
-    launch_gather(ctx, d.recs, words, n, pl, zmin, static_cast<Row64*>(cat->row), cat->zoff,
-                  cat->n_zoff, words_alt);
+    launch_gather(ctx, d.recs, words, n, pl, ra_drop, zmin, static_cast<Row64*>(cat->row),
+                  cat->zoff, cat->n_zoff, words_alt);
     dfree(ctx, words);
     cat->key = words_alt;
 }

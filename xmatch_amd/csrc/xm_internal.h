@@ -210,9 +210,18 @@ constexpr size_t PIN_SORT_ERR_OFFSET = 1024;
 // range and bad count, in one pass.
 // With n_pass > 0 it also adds the counts of every digit value at digit positions
 // lo_bit + 8p (p < n_pass) of the valid records' words into digit_totals[p][256].
+// With ra_drop (device word) the digit positions start ra_drop bits higher and
+// cover only the passes the sort then needs.
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad,
-                      int lo_bit = 0, int n_pass = 0, uint32_t* digit_totals = nullptr);
+                      int lo_bit = 0, int n_pass = 0, uint32_t* digit_totals = nullptr,
+                      const uint32_t* ra_drop = nullptr);
+// One-segment key layout: from a strided sample of the records, the number of low
+// ra bits the sort and the catalog key drop, written to *drop_out (device; stats is
+// scratch of LAYOUT_STATS_WORDS words).  TPB must be 256 (one thread per bin).
+constexpr int LAYOUT_STATS_WORDS = 516;
+void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
+                   int ra_bits, uint32_t* stats, uint32_t* drop_out);
 // Many-segment build, pass 1: zone range and bad count of every segment
 // (range[3 s] = zmin, range[3 s + 1] = zmax, range[3 s + 2] = bad).
 void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range);
@@ -229,8 +238,8 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 // one-segment form takes the layout by value; the many-segment form reads it
 // from the segment table.
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
-                   PackLayout pl, int32_t zmin, Row64* rows, uint32_t* zoff, uint64_t n_zoff,
-                   uint64_t* keys);
+                   PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, uint32_t* zoff,
+                   uint64_t n_zoff, uint64_t* keys);
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
                        uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys);
 // counts[k] = chunks of zoff entry k (k < n_entries); chunks from the scanned counts.
