@@ -301,6 +301,7 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
 // the pair's rows give the ids and the FP64 chord -- the same chord2<double> of the
 // same two unit vectors the pair kernel tested, so the bits agree.  chord^2 turns
 // into 2 asin(sqrt(d2)/2) in arcsec.  The part prefix is searched in LDS.
+constexpr uint32_t COMPACT_ITEMS = 4;  // measured on 30M matches: 1 -> 0.329 ms, 4 -> 0.267, 16 -> 0.275
 __global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
                                                  const uint64_t* __restrict__ prefix,
                                                  const uint64_t* __restrict__ base,
@@ -324,27 +325,31 @@ __global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
         s_base[r] = base[r];
     }
     __syncthreads();
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= total) return;
-    uint32_t lo = 0, hi = n_parts - 1;  // last part with prefix <= i
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (s_pre[mid] <= i) lo = mid;
-        else hi = mid - 1;
+    // COMPACT_ITEMS outputs per thread, so each block's load of the part tables
+    // serves 256 * COMPACT_ITEMS outputs
+    for (uint32_t k = 0; k < COMPACT_ITEMS; k++) {
+        const uint64_t i = (uint64_t(blockIdx.x) * COMPACT_ITEMS + k) * 256 + threadIdx.x;
+        if (i >= total) return;
+        uint32_t lo = 0, hi = n_parts - 1;  // last part with prefix <= i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (s_pre[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint64_t src = s_base[lo] + (i - s_pre[lo]);
+        if (wa) {
+            id_a[i] = wa[src];
+            id_b[i] = wb[src];
+            dist[i] = dist_arcsec(wd[src]);
+        } else {
+            const Row64 a = arow[ta[src]];
+            const Row64 b = brow[tb[src]];
+            id_a[i] = a.id;
+            id_b[i] = b.id;
+            dist[i] = dist_arcsec(chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z));
+        }
+        if (job) job[i] = tj[src];
     }
-    const uint64_t src = s_base[lo] + (i - s_pre[lo]);
-    if (wa) {
-        id_a[i] = wa[src];
-        id_b[i] = wb[src];
-        dist[i] = dist_arcsec(wd[src]);
-    } else {
-        const Row64 a = arow[ta[src]];
-        const Row64 b = brow[tb[src]];
-        id_a[i] = a.id;
-        id_b[i] = b.id;
-        dist[i] = dist_arcsec(chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z));
-    }
-    if (job) job[i] = tj[src];
 }
 
 // --------------------------------------------------------------- sparse ----
@@ -745,7 +750,9 @@ void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, ui
                     double* dist, uint32_t* job) {
     if (total == 0) return;
     launch_timed(ctx, "compact", [&] {
-        hipLaunchKernelGGL(k_compact, dim3(uint32_t((total + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_compact,
+                           dim3(uint32_t((total + 256 * COMPACT_ITEMS - 1) / (256 * COMPACT_ITEMS))),
+                           dim3(256), 0,
                            ctx->stream, ro.n_regions + SPILL_POOLS, prefix, ro.base, total,
                            (const int64_t*)(ro.wide ? ro.id_a : nullptr),
                            (const int64_t*)ro.id_b, (const double*)ro.d2,
