@@ -278,9 +278,7 @@ struct Ctx {
 };
 
 // Reads records [first, first + n) of a file into dst (pread, any length).
-bool read_records(int fd, uint64_t first, uint64_t n, xm_obj* dst) {
-    uint64_t bytes = n * sizeof(xm_obj), off = first * sizeof(xm_obj);
-    char* p = reinterpret_cast<char*>(dst);
+bool read_span(int fd, uint64_t off, uint64_t bytes, char* p) {
     while (bytes) {
         const ssize_t got = pread(fd, p, std::min<uint64_t>(bytes, uint64_t(1) << 30), off_t(off));
         if (got <= 0) return false;
@@ -289,6 +287,105 @@ bool read_records(int fd, uint64_t first, uint64_t n, xm_obj* dst) {
         bytes -= uint64_t(got);
     }
     return true;
+}
+
+bool write_span(int fd, uint64_t off, uint64_t bytes, const char* p) {
+    while (bytes) {
+        const ssize_t put = pwrite(fd, p, std::min<uint64_t>(bytes, uint64_t(1) << 30), off_t(off));
+        if (put <= 0) return false;
+        p += put;
+        off += uint64_t(put);
+        bytes -= uint64_t(put);
+    }
+    return true;
+}
+
+// Records [first, first + n) of the file, read by up to READ_THREADS threads in
+// parallel spans (one thread copies from the page cache at a few GB/s).
+constexpr unsigned READ_THREADS = 8;
+bool read_records(int fd, uint64_t first, uint64_t n, xm_obj* dst) {
+    const uint64_t bytes = n * sizeof(xm_obj), off = first * sizeof(xm_obj);
+    char* p = reinterpret_cast<char*>(dst);
+    const uint64_t min_span = uint64_t(64) << 20;
+    const unsigned k = unsigned(std::min<uint64_t>(READ_THREADS, std::max<uint64_t>(1, bytes / min_span)));
+    if (k <= 1) return read_span(fd, off, bytes, p);
+    const uint64_t span = (bytes + k - 1) / k;
+    std::vector<std::thread> th;
+    std::vector<char> ok(k, 0);
+    for (unsigned t = 0; t < k; t++) {
+        const uint64_t a = std::min<uint64_t>(bytes, t * span), b = std::min<uint64_t>(bytes, a + span);
+        th.emplace_back([&, t, a, b] { ok[t] = read_span(fd, off + a, b - a, p + a); });
+    }
+    for (auto& x : th) x.join();
+    for (char c : ok)
+        if (!c) return false;
+    return true;
+}
+
+// Round r's pairs to <path>: copied back into page-locked memory, then formatted
+// and written by up to WRITE_THREADS threads, each into its own slice of the file
+// (records are fixed-size), under a temporary name that is renamed when the file
+// is complete (what -c relies on).
+constexpr unsigned WRITE_THREADS = 8;
+bool write_round(const Ctx& C, const xm_result* res, uint64_t m, const std::string& path,
+                 int format) {
+    int64_t* ia = nullptr;
+    int64_t* ib = nullptr;
+    double* dist = nullptr;
+    const uint64_t n_alloc = std::max<uint64_t>(m, 1);
+    C.check(xm_host_alloc(C.c, n_alloc * 8, reinterpret_cast<void**>(&ia)), "pinned output");
+    C.check(xm_host_alloc(C.c, n_alloc * 8, reinterpret_cast<void**>(&ib)), "pinned output");
+    C.check(xm_host_alloc(C.c, n_alloc * 8, reinterpret_cast<void**>(&dist)), "pinned output");
+    C.check(xm_result_copy(C.c, res, ia, ib, dist), "copy");
+    const std::string part = path + ".part";
+    const int fd = open(part.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0666);
+    bool ok = fd >= 0;
+    if (!ok) std::cerr << "xmatch: cannot write " << part << std::endl;
+    const size_t rec = size_t(format);
+    if (ok && m) {
+        const uint64_t min_rows = uint64_t(1) << 20;
+        const unsigned k = unsigned(std::min<uint64_t>(WRITE_THREADS, std::max<uint64_t>(1, m / min_rows)));
+        const uint64_t per = (m + k - 1) / k;
+        std::vector<std::thread> th;
+        std::vector<char> good(k, 1);
+        for (unsigned t = 0; t < k; t++) {
+            th.emplace_back([&, t] {
+                const uint64_t a = std::min<uint64_t>(m, t * per), b = std::min<uint64_t>(m, a + per);
+                const uint64_t chunk = uint64_t(1) << 18;
+                std::vector<char> buf(size_t(std::min<uint64_t>(b - a, chunk)) * rec + 1);
+                for (uint64_t k0 = a; k0 < b; k0 += chunk) {
+                    const uint64_t k1 = std::min<uint64_t>(b, k0 + chunk);
+                    char* q = buf.data();
+                    for (uint64_t i = k0; i < k1; i++, q += rec) {
+                        std::memcpy(q, &ia[i], 8);
+                        std::memcpy(q + 8, &ib[i], 8);
+                        if (rec == 24) std::memcpy(q + 16, &dist[i], 8);
+                    }
+                    if (!write_span(fd, k0 * rec, (k1 - k0) * rec, buf.data())) {
+                        good[t] = 0;
+                        return;
+                    }
+                }
+            });
+        }
+        for (auto& x : th) x.join();
+        for (char g : good) ok = ok && g;
+        if (!ok) std::cerr << "xmatch: write error on " << path << std::endl;
+    }
+    if (fd >= 0 && close(fd) != 0) ok = false;
+    if (ok && std::rename(part.c_str(), path.c_str()) != 0) {
+        std::cerr << "xmatch: cannot finish " << path << std::endl;
+        ok = false;
+    }
+    xm_host_free(C.c, ia);
+    xm_host_free(C.c, ib);
+    xm_host_free(C.c, dist);
+    return ok;
+}
+
+// Seconds since t0, for the -v 2 phase times.
+double since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Segments of at most num_obj rows over [0, n) of a device record array.
@@ -333,31 +430,8 @@ int main(int argc, char** argv) {
         std::cerr << "xmatch: cannot open inputs" << std::endl;
         return 3;
     }
-    // ---- file A: resident on the device as one many-segment catalog
-    xm_catalog* A = nullptr;
-    {
-        xm_obj* h = nullptr;
-        void* d = nullptr;
-        C.check(xm_host_alloc(C.c, nA * sizeof(xm_obj), reinterpret_cast<void**>(&h)), "pinned A");
-        if (!read_records(fa, 0, nA, h)) {
-            std::cerr << "xmatch: read error on " << o.inputs[0] << std::endl;
-            return 3;
-        }
-        C.check(xm_device_alloc(C.c, nA * sizeof(xm_obj), &d), "device A");
-        C.check(xm_memcpy_h2d(C.c, d, h, nA * sizeof(xm_obj)), "upload A");
-        C.check(xm_host_free(C.c, h), "pinned A");
-        auto segs = cut(static_cast<xm_obj*>(d), nA, o.num_obj, zh_deg);
-        if (segs.size() > 65536) {
-            std::cerr << "xmatch: file A makes more than 65536 segments; raise -n" << std::endl;
-            return 1;
-        }
-        C.check(xm_catalog_create_segments_device(C.c, segs.data(), segs.size(), &A), "prepare A");
-        C.check(xm_device_free(C.c, d), "device A");
-        if (o.verbosity) std::cerr << "A: " << segs.size() << " segment(s) prepared" << std::endl;
-    }
-    uint64_t nsegA = 0;
-    xm_catalog_segment_count(A, &nsegA);
-    // ---- file B in rounds: whole segments, the next round read ahead
+    const double t_init = since(t_start);
+    // ---- the rounds of file B: whole segments, the next round read ahead
     uint64_t round_rows = o.round_rows;
     if (!round_rows) {
         // a round needs its records, catalog build and output (~200 B per row);
@@ -371,12 +445,51 @@ int main(int argc, char** argv) {
     round_rows = std::min<uint64_t>(round_rows / seg_b * seg_b, 0xfffffffeull / seg_b * seg_b);
     std::vector<std::pair<uint64_t, uint64_t>> rounds;  // (first row, rows)
     for (uint64_t at = 0; at < nB; at += round_rows) rounds.push_back({at, std::min(round_rows, nB - at)});
-
+    // Two page-locked buffers: file A goes through hb[0]; round r of B through
+    // hb[(r + 1) & 1], so round 0 is read while A uploads and is prepared, and each
+    // later round while the one before it joins.
+    const uint64_t b_rows = rounds.empty() ? 1 : rounds[0].second;
     xm_obj* hb[2] = {nullptr, nullptr};
-    const uint64_t hb_rows = rounds.empty() ? 1 : rounds[0].second;
-    for (auto& p : hb)
-        C.check(xm_host_alloc(C.c, hb_rows * sizeof(xm_obj), reinterpret_cast<void**>(&p)), "pinned B");
-    bool read_ok = read_records(fb, rounds[0].first, rounds[0].second, hb[0]);
+    C.check(xm_host_alloc(C.c, std::max(nA, b_rows) * sizeof(xm_obj), reinterpret_cast<void**>(&hb[0])),
+            "pinned A");
+    C.check(xm_host_alloc(C.c, b_rows * sizeof(xm_obj), reinterpret_cast<void**>(&hb[1])), "pinned B");
+    const double t_pin = since(t_start);
+    bool read_ok = true;
+    std::thread b0_reader;
+    if (!rounds.empty())
+        b0_reader = std::thread([&] { read_ok = read_records(fb, rounds[0].first, rounds[0].second, hb[1]); });
+    // ---- file A: resident on the device as one many-segment catalog
+    xm_catalog* A = nullptr;
+    double t_read_a = 0, t_prep_a = 0;
+    {
+        void* d = nullptr;
+        if (!read_records(fa, 0, nA, hb[0])) {
+            std::cerr << "xmatch: read error on " << o.inputs[0] << std::endl;
+            if (b0_reader.joinable()) b0_reader.join();
+            return 3;
+        }
+        t_read_a = since(t_start);
+        C.check(xm_device_alloc(C.c, nA * sizeof(xm_obj), &d), "device A");
+        C.check(xm_memcpy_h2d(C.c, d, hb[0], nA * sizeof(xm_obj)), "upload A");
+        auto segs = cut(static_cast<xm_obj*>(d), nA, o.num_obj, zh_deg);
+        if (segs.size() > 65536) {
+            std::cerr << "xmatch: file A makes more than 65536 segments; raise -n" << std::endl;
+            if (b0_reader.joinable()) b0_reader.join();
+            return 1;
+        }
+        C.check(xm_catalog_create_segments_device(C.c, segs.data(), segs.size(), &A), "prepare A");
+        C.check(xm_device_free(C.c, d), "device A");
+        t_prep_a = since(t_start);
+        if (o.verbosity) std::cerr << "A: " << segs.size() << " segment(s) prepared" << std::endl;
+    }
+    uint64_t nsegA = 0;
+    xm_catalog_segment_count(A, &nsegA);
+    if (b0_reader.joinable()) b0_reader.join();
+    const double t_read_b0 = since(t_start);
+    if (o.verbosity >= 2)
+        std::cerr << "times (s from start): context " << t_init << ", page-locked buffers " << t_pin
+                  << ", A read " << t_read_a << ", A prepared " << t_prep_a
+                  << ", B round 0 read " << t_read_b0 << std::endl;
     uint64_t total = 0, skipped = 0;
     for (size_t r = 0; r < rounds.size() && read_ok; r++) {
         bool next_ok = true;
@@ -388,18 +501,17 @@ int main(int argc, char** argv) {
             if (o.verbosity) std::cerr << "round " << r << ": done before, skipped" << std::endl;
             skipped++;
             if (r + 1 < rounds.size())
-                read_ok = read_records(fb, rounds[r + 1].first, rounds[r + 1].second,
-                                       hb[(r + 1) & 1]);
+                read_ok = read_records(fb, rounds[r + 1].first, rounds[r + 1].second, hb[r & 1]);
             continue;
         }
         if (r + 1 < rounds.size())
             reader = std::thread([&, r] {
-                next_ok = read_records(fb, rounds[r + 1].first, rounds[r + 1].second, hb[(r + 1) & 1]);
+                next_ok = read_records(fb, rounds[r + 1].first, rounds[r + 1].second, hb[r & 1]);
             });
         const uint64_t n = rounds[r].second;
         void* d = nullptr;
         C.check(xm_device_alloc(C.c, n * sizeof(xm_obj), &d), "device B");
-        C.check(xm_memcpy_h2d(C.c, d, hb[r & 1], n * sizeof(xm_obj)), "upload B");
+        C.check(xm_memcpy_h2d(C.c, d, hb[(r + 1) & 1], n * sizeof(xm_obj)), "upload B");
         auto segs = cut(static_cast<xm_obj*>(d), n, o.num_obj, zh_deg);
         xm_catalog* B = nullptr;
         C.check(xm_catalog_create_segments_device(C.c, segs.data(), segs.size(), &B), "prepare B");
@@ -416,38 +528,7 @@ int main(int argc, char** argv) {
         uint64_t m = 0;
         xm_result_count(res, &m);
         total += m;
-        if (!o.outdir.empty()) {
-            std::vector<int64_t> ia(m), ib(m);
-            std::vector<double> dist(m);
-            C.check(xm_result_copy(C.c, res, ia.data(), ib.data(), dist.data()), "copy");
-            // written under a temporary name and renamed when complete, so a round's
-            // file exists only whole (what -c relies on)
-            const std::string part = path + ".part";
-            FILE* f = std::fopen(part.c_str(), "wb");
-            if (!f) {
-                std::cerr << "xmatch: cannot write " << part << std::endl;
-                return 4;
-            }
-            const size_t rec = size_t(o.format), chunk = size_t(1) << 20;
-            std::vector<char> buf(std::min<uint64_t>(m, chunk) * rec + 1);
-            for (uint64_t k0 = 0; k0 < m; k0 += chunk) {
-                const uint64_t k1 = std::min<uint64_t>(m, k0 + chunk);
-                char* p = buf.data();
-                for (uint64_t k = k0; k < k1; k++, p += rec) {
-                    std::memcpy(p, &ia[k], 8);
-                    std::memcpy(p + 8, &ib[k], 8);
-                    if (rec == 24) std::memcpy(p + 16, &dist[k], 8);
-                }
-                if (std::fwrite(buf.data(), rec, k1 - k0, f) != k1 - k0) {
-                    std::cerr << "xmatch: write error on " << path << std::endl;
-                    return 4;
-                }
-            }
-            if (std::fclose(f) != 0 || std::rename(part.c_str(), path.c_str()) != 0) {
-                std::cerr << "xmatch: cannot finish " << path << std::endl;
-                return 4;
-            }
-        }
+        if (!o.outdir.empty() && !write_round(C, res, m, path, o.format)) return 4;
         if (o.verbosity)
             std::cerr << "round " << r << ": " << n << " B records, " << jobs.size() << " jobs, " << m
                       << " matches, join " << st.join_ms << " ms" << std::endl;
