@@ -29,6 +29,8 @@ import xmatch_amd as xm  # noqa: E402
 from xmatch_amd import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# untimed steps with events around every launch, for the per-kernel breakdown
+BREAKDOWN_STEPS = 3
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 # algorithmic HBM bytes per source for the memory-bound kernels of a catalog build
@@ -497,9 +499,26 @@ def main():
     for i in range(args.warmup):
         step()
         log("warmup step %d done" % i)
+    # Per-kernel breakdown: events around every launch, on steps of their own.  Each
+    # timed launch adds two event records to the stream (~0.38 ms over the 45
+    # launches of a step), so the timed steps below carry events around the
+    # dominant kernel only.
     ctx.synchronize()
     ctx.reset_timing()
     ctx.set_timing(True)
+    for i in range(BREAKDOWN_STEPS):
+        step()
+    ctx.set_timing(False)
+    ktimes = {}
+    for k in KERNELS:
+        tot, cnt = ctx.kernel_time(k)
+        if cnt:
+            ktimes[k] = {"total_ms": tot, "launches": cnt, "avg_ms": tot / cnt}
+    dom = max(ktimes.items(), key=lambda kv: kv[1]["total_ms"])[0]
+    log("breakdown over %d steps; dominant kernel %s" % (BREAKDOWN_STEPS, dom))
+
+    ctx.reset_timing()
+    ctx.set_timing(True, only=dom)
     ctx.synchronize()
     t_start = time.perf_counter()
     join_ms = 0.0
@@ -511,12 +530,8 @@ def main():
     elapsed = time.perf_counter() - t_start
     ctx.set_timing(False)
     log("timed %d steps in %.3f s" % (args.steps, elapsed))
-
-    ktimes = {}
-    for k in KERNELS:
-        tot, cnt = ctx.kernel_time(k)
-        if cnt:
-            ktimes[k] = {"total_ms": tot, "launches": cnt, "avg_ms": tot / cnt}
+    dom_total_ms, dom_launches = ctx.kernel_time(dom)
+    dom_avg_ms = dom_total_ms / dom_launches
 
     ca = ctx.catalog(dA, r)
     cb = ctx.catalog(dB, r)
@@ -528,24 +543,26 @@ def main():
     value = w["w_cand"] / (ms_per_step / 1e3)
     join_ms_avg = join_ms / args.steps
 
-    # dominant kernel by total device time
-    dom = max(ktimes.items(), key=lambda kv: kv[1]["total_ms"])[0]
+    # the dominant kernel (by total device time in the breakdown), its average
+    # launch duration timed over the timed steps
     if dom == "pair_fp64":
         # 9 FP64 VALU operations per executed pair test (3 sub, 3 mul, 2 add, 1 cmp)
         flops = w["w_tile"] * 9.0
-        ach = flops / (ktimes[dom]["avg_ms"] / 1e3) / 1e12
+        ach = flops / (dom_avg_ms / 1e3) / 1e12
         roof = {"kernel": dom, "bound": "valu", "achieved": ach, "peak": FP64_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP64_VECTOR_PEAK_TFLOPS, "traffic": None}
     else:
         per_launch = BYTES_PER_SOURCE.get(dom, 0.0) * args.n
-        ach = per_launch / (ktimes[dom]["avg_ms"] / 1e3) / 1e9
+        ach = per_launch / (dom_avg_ms / 1e3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                 "algorithmic_bytes_per_launch": per_launch}
+    roof["avg_launch_ms"] = dom_avg_ms
+    roof["launches_timed"] = dom_launches
 
     sort_ms = sum(ktimes.get(k, {}).get("total_ms", 0.0)
-                  for k in ("radix_hist", "radix_offsets", "radix_scatter")) / args.steps
-    passes = ktimes.get("radix_scatter", {}).get("launches", 0) / max(1, args.steps) / 2
+                  for k in ("radix_hist", "radix_offsets", "radix_scatter")) / BREAKDOWN_STEPS
+    passes = ktimes.get("radix_scatter", {}).get("launches", 0) / BREAKDOWN_STEPS / 2
     # per catalog: histogram read 8, then per pass scatter read 8 + write 8; both catalogs
     sort_bytes = 2 * args.n * (8 + passes * 16)
 
@@ -562,7 +579,9 @@ def main():
         "w_ref": w["w_ref"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
         "sort_ms_per_step": sort_ms, "sort_passes_per_catalog": passes,
         "sort_gb_per_s": sort_bytes / (sort_ms / 1e3) / 1e9 if sort_ms else None,
-        "kernels": ktimes, "roofline": roof,
+        "kernels": ktimes, "kernels_from": "%d untimed steps with events around every launch"
+                                             % BREAKDOWN_STEPS,
+        "roofline": roof,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

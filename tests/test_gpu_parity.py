@@ -130,6 +130,32 @@ def test_sort_lookback_failure_is_reported(gpu_ctx):
     assert_same(got, ref)
 
 
+def test_timing_only_one_kernel(gpu_ctx):
+    # Timing with a kernel named records that kernel's launches and no other; with
+    # none named, every kernel's.
+    A, _, r = synth.config_small_patch(seed=3, n_a=40_000, n_b=10)
+    try:
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True, only="gather")
+        gpu_ctx.catalog(A, r).close()
+        gpu_ctx.catalog(A, r).close()
+        gpu_ctx.set_timing(False)
+        t, c = gpu_ctx.kernel_time("gather")
+        assert c == 2 and t > 0
+        assert gpu_ctx.kernel_time("make_keys")[1] == 0
+        assert gpu_ctx.kernel_time("radix_scatter")[1] == 0
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True)
+        gpu_ctx.catalog(A, r).close()
+        gpu_ctx.set_timing(False)
+        assert gpu_ctx.kernel_time("gather")[1] == 1
+        assert gpu_ctx.kernel_time("make_keys")[1] == 1
+        assert gpu_ctx.kernel_time("radix_scatter")[1] >= 1
+    finally:
+        gpu_ctx.set_timing(False)
+        gpu_ctx.reset_timing()
+
+
 @pytest.mark.parametrize("order", ["dec", "dec_ra", "reversed"])
 def test_ordered_input_same_set(gpu_ctx, order):
     # Spatially ordered input (as survey catalogs are often stored) gives whole

@@ -32,7 +32,7 @@ _STATUS = {0: "XM_OK", 1: "XM_EINVAL", 2: "XM_ENOMEM", 3: "XM_EDEVICE", 4: "XM_E
 # every symbol declared in include/xmatch.h
 EXPORTED = [
     "xm_abi_version", "xm_ctx_create", "xm_ctx_destroy", "xm_last_error_message",
-    "xm_ctx_set_timing", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
+    "xm_ctx_set_timing", "xm_ctx_set_timing_only", "xm_ctx_kernel_time", "xm_ctx_reset_timing", "xm_ctx_synchronize",
     "xm_ctx_set_param", "xm_ctx_get_param", "xm_ctx_release_memory",
     "xm_device_alloc", "xm_device_free", "xm_memcpy_h2d", "xm_memcpy_d2h",
     "xm_host_alloc", "xm_host_free", "xm_ctx_mem_info",
@@ -149,6 +149,7 @@ def lib():
             "xm_ctx_destroy": ([vp], None),
             "xm_last_error_message": ([vp], ctypes.c_char_p),
             "xm_ctx_set_timing": ([vp, ctypes.c_int], ctypes.c_int),
+            "xm_ctx_set_timing_only": ([vp, ctypes.c_char_p], ctypes.c_int),
             "xm_ctx_kernel_time": ([vp, ctypes.c_char_p, ctypes.POINTER(dbl),
                                     ctypes.POINTER(u64)], ctypes.c_int),
             "xm_ctx_reset_timing": ([vp], ctypes.c_int),
@@ -248,7 +249,9 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def set_timing(self, on=True):
+    def set_timing(self, on=True, only=None):
+        """Per-launch event timing; `only` names the one kernel to time (None: all)."""
+        self._check(lib().xm_ctx_set_timing_only(self._h, only.encode() if only else None))
         self._check(lib().xm_ctx_set_timing(self._h, 1 if on else 0))
 
     def kernel_time(self, name):

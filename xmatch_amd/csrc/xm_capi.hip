@@ -570,6 +570,12 @@ xm_status xm_ctx_set_timing(xm_ctx* ctx, int enable) {
     return guarded(ctx, [&] { ctx->timing = enable != 0; });
 }
 
+xm_status xm_ctx_set_timing_only(xm_ctx* ctx, const char* name) {
+    if (!ctx) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] { ctx->timing_only = name ? name : ""; });
+}
+
 xm_status xm_ctx_kernel_time(xm_ctx* ctx, const char* name, double* total_ms, uint64_t* count) {
     if (!ctx || !name) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);

@@ -46,6 +46,7 @@ struct xm_ctx {
     std::mutex mu;
     std::string last_error = "no error";
     bool timing = false;
+    std::string timing_only;                 // time only this kernel (empty: all)
     std::vector<xm::TimedLaunch> launches;   // pending, resolved on query
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
@@ -156,10 +157,11 @@ void timing_end(xm_ctx* ctx, const char* name, hipEvent_t ev);
 template <typename F>
 inline void launch_timed(xm_ctx* ctx, const char* name, F&& f) {
     hipEvent_t ev = nullptr;
-    if (ctx->timing) timing_begin(ctx, name, &ev);
+    const bool timed = ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == name);
+    if (timed) timing_begin(ctx, name, &ev);
     f();
     XM_HIP(hipGetLastError());
-    if (ctx->timing) timing_end(ctx, name, ev);
+    if (timed) timing_end(ctx, name, ev);
 }
 
 // ---- scan / sort (xm_sort.hip)
