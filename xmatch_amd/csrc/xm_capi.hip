@@ -323,20 +323,20 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     d.zone_base = 0;
     uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
     uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-    int32_t* small = static_cast<int32_t*>(dalloc(ctx, 4 * sizeof(int32_t)));
-    launch_fill_i32(ctx, small, 1, INT_MAX);
-    launch_fill_i32(ctx, small + 1, 1, -1);
-    launch_fill_i32(ctx, small + 2, 1, 0);
-    // the ra bits the sort and the key drop, decided on the device from a sample
-    uint32_t* stats = static_cast<uint32_t*>(dalloc(ctx, LAYOUT_STATS_WORDS * sizeof(uint32_t)));
-    launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, stats,
-                  reinterpret_cast<uint32_t*>(small + 3));
     // the sort's digit counts, taken while the keys are written: digit positions
     // from the row bits up over the ra bits and every zone bit (the sort may need
     // fewer once the occupied zone range is known)
     const int hist_pass = (pl.ra_bits + pl.zone_bits + 7) / 8;
-    uint32_t* digit_totals = static_cast<uint32_t*>(dalloc(ctx, size_t(hist_pass) * 256 * 4));
-    launch_fill_u32(ctx, digit_totals, uint64_t(hist_pass) * 256, 0u);
+    // scratch: zone range, bad count and dropped ra bits; layout statistics; digit counts
+    uint32_t* scratch = static_cast<uint32_t*>(dalloc(
+        ctx, (BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + size_t(hist_pass) * 256) * sizeof(uint32_t)));
+    launch_build_init(ctx, scratch, uint64_t(hist_pass) * 256);
+    int32_t* small = reinterpret_cast<int32_t*>(scratch);
+    uint32_t* stats = scratch + BUILD_SMALL_WORDS;
+    uint32_t* digit_totals = stats + LAYOUT_STATS_WORDS;
+    // the ra bits the sort and the key drop, decided on the device from a sample
+    launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, stats,
+                  reinterpret_cast<uint32_t*>(small + 3));
     launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
                      reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
                      digit_totals, reinterpret_cast<uint32_t*>(small + 3));
@@ -346,12 +346,10 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     const int32_t zmin = pin[0], zmax = pin[1];
     const uint32_t nbad = uint32_t(pin[2]);
     const int ra_drop = pin[3];
-    dfree(ctx, small);
-    dfree(ctx, stats);
     d.ra_bits = pl.ra_bits - ra_drop;
     d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     if (nbad) {
-        dfree(ctx, digit_totals);
+        dfree(ctx, scratch);
         dfree(ctx, words);
         dfree(ctx, words_alt);
         throw InvalidArg{std::to_string(nbad) +
@@ -366,7 +364,7 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
     radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals);
-    dfree(ctx, digit_totals);
+    dfree(ctx, scratch);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     launch_gather(ctx, d.recs, words, n, pl, ra_drop, zmin, static_cast<Row64*>(cat->row),

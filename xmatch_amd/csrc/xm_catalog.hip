@@ -56,7 +56,6 @@ constexpr uint32_t LAYOUT_SAMPLE = 16384;
 constexpr double LAYOUT_MAX_EXTRA = 0.1;
 // stats: [0, 256) zone bins, [256, 512) ra bins, [512] zmin, [513] zmax,
 // [514] sampled records, [515] ra bits to drop
-constexpr int LAYOUT_STATS = LAYOUT_STATS_WORDS;
 static_assert(TPB == 256, "the layout kernels take one thread per histogram bin");
 
 __global__ __launch_bounds__(TPB) void k_layout_sample(const xm_obj* __restrict__ recs, uint64_t n,
@@ -456,6 +455,20 @@ __global__ __launch_bounds__(TPB) void k_fill(T* __restrict__ p, uint64_t n, T v
 
 inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 
+// The one-segment build's scratch words, set in one launch rather than one fill
+// each: small = {INT_MAX, -1, 0, 0} (zone min, zone max, bad records, ra bits
+// dropped), then the layout statistics (zero but zone min INT_MAX and zone max -1),
+// then the digit counters (zero).
+__global__ __launch_bounds__(TPB) void k_build_init(uint32_t* __restrict__ w, uint32_t n_words) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n_words) return;
+    constexpr uint32_t zmin_at = BUILD_SMALL_WORDS + 512, zmax_at = BUILD_SMALL_WORDS + 513;
+    uint32_t v = 0;
+    if (i == 0 || i == zmin_at) v = 0x7fffffffu;
+    else if (i == 1 || i == zmax_at) v = 0xffffffffu;
+    w[i] = v;
+}
+
 }  // namespace
 
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
@@ -472,11 +485,16 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
     });
 }
 
+void launch_build_init(xm_ctx* ctx, uint32_t* scratch, uint64_t n_digit_words) {
+    const uint64_t n = BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + n_digit_words;
+    launch_timed(ctx, "fill", [&] {
+        hipLaunchKernelGGL(k_build_init, grid_for(n), dim3(TPB), 0, ctx->stream, scratch,
+                           uint32_t(n));
+    });
+}
+
 void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                    int ra_bits, uint32_t* stats, uint32_t* drop_out) {
-    launch_fill_u32(ctx, stats, LAYOUT_STATS, 0u);
-    launch_fill_i32(ctx, reinterpret_cast<int32_t*>(stats + 512), 1, INT_MAX);
-    launch_fill_i32(ctx, reinterpret_cast<int32_t*>(stats + 513), 1, -1);
     if (n == 0) return;
     const uint32_t m = uint32_t(std::min<uint64_t>(n, LAYOUT_SAMPLE));
     launch_timed(ctx, "layout", [&] {

@@ -220,10 +220,17 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
                       const uint32_t* ra_drop = nullptr);
 // One-segment key layout: from a strided sample of the records, the number of low
 // ra bits the sort and the catalog key drop, written to *drop_out (device; stats is
-// scratch of LAYOUT_STATS_WORDS words).  TPB must be 256 (one thread per bin).
+// scratch of LAYOUT_STATS_WORDS words, set up by launch_build_init).  TPB must be
+// 256 (one thread per bin).
 constexpr int LAYOUT_STATS_WORDS = 516;
 void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                    int ra_bits, uint32_t* stats, uint32_t* drop_out);
+// The one-segment build's scratch, BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS +
+// n_digit_words words, set up by one launch: small words {zone min = INT_MAX, zone
+// max = -1, bad records = 0, ra bits dropped = 0}, the layout statistics, the sort's
+// digit counters (zero).
+constexpr int BUILD_SMALL_WORDS = 4;
+void launch_build_init(xm_ctx* ctx, uint32_t* scratch, uint64_t n_digit_words);
 // Many-segment build, pass 1: zone range and bad count of every segment
 // (range[3 s] = zmin, range[3 s + 1] = zmax, range[3 s + 2] = bad).
 void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range);
