@@ -246,7 +246,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __shared__ uint64_t s_k[RS_TILE];
     __shared__ uint32_t s_wc[RS_WAVES][256];   // per-wave running digit counts, then prefixes
     __shared__ uint32_t s_hist[256];    // the tile's digit counts
-    __shared__ uint32_t s_ts[256];      // tile-local start of each digit
     __shared__ uint32_t s_go[256];      // global start minus tile-local start of each digit
     __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
     __shared__ uint32_t s_tile;
@@ -330,19 +329,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             s_mask[w * 256 + d] = 0;
         }
     }
-    __syncthreads();
+    // tile-local start of each digit; then s_wc[q][d] becomes the tile-local
+    // position of wavefront q's first key of digit d, so the regroup reads one word
+    // per key
+    const uint32_t ts = block_excl_scan<RS_THREADS>(cnt, s_w);
     if (dig) {
-        uint32_t run = 0;
+        uint32_t run = ts;
 #pragma unroll
         for (int q = 0; q < RS_WAVES; q++) {
             const uint32_t c = s_wc[q][tid];
             s_wc[q][tid] = run;
             run += c;
         }
-    }
-    {
-        const uint32_t ts = block_excl_scan<RS_THREADS>(cnt, s_w);
-        if (dig) s_ts[tid] = ts;
     }
     __syncthreads();
     // regroup by digit in LDS (tile-local positions only)
@@ -351,7 +349,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         const uint32_t i = base + k * WAVE + lane;
         if (i < n) {
             const uint32_t d = uint32_t(key[k] >> shift) & 255u;
-            s_k[s_ts[d] + s_wc[w][d] + rk[k]] = key[k];
+            s_k[s_wc[w][d] + rk[k]] = key[k];
         }
     }
     // look-back: sum the counts published by the earlier tiles, nearest first, up
@@ -393,7 +391,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             __hip_atomic_store(mine, LB::make(parity, true, excl + cnt), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (dig) s_go[tid] = digit_start[tid] + excl - s_ts[tid];
+    if (dig) s_go[tid] = digit_start[tid] + excl - ts;
     __syncthreads();
     const uint32_t tile0 = tile * RS_TILE;
     const uint32_t tile_n = (n - tile0) < uint32_t(RS_TILE) ? (n - tile0) : uint32_t(RS_TILE);
