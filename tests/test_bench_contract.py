@@ -1,0 +1,27 @@
+"""The parts of bench.py's output contract that need no GPU: it refuses more than
+one GPU with a message, and the roofline's traffic figure is read from the
+committed rocprofv3 summary of the headline workload."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_traffic_is_read_from_the_committed_profile():
+    import bench
+    t = bench.measured_traffic("gather")
+    # FETCH_SIZE x 2.0 + WRITE_SIZE of the 50M-row gather: several GB per launch,
+    # above its 72 B/row algorithmic bytes (the random record reads fetch whole lines)
+    assert t is not None and t > bench.BYTES_PER_SOURCE["gather"] * 50_000_000
+    assert bench.measured_traffic("radix_scatter") > 0
+    assert bench.measured_traffic("no_such_kernel") is None
