@@ -500,9 +500,9 @@ def main():
         step()
         log("warmup step %d done" % i)
     # Per-kernel breakdown: events around every launch, on steps of their own.  Each
-    # timed launch adds two event records to the stream (~0.38 ms over the 45
-    # launches of a step), so the timed steps below carry events around the
-    # dominant kernel only.
+    # timed launch adds two event records to the stream (~8 us per launch on
+    # MI355X), so the timed steps below carry events around the dominant kernel
+    # only.
     ctx.synchronize()
     ctx.reset_timing()
     ctx.set_timing(True)

@@ -91,9 +91,9 @@ const char *xm_last_error_message(const xm_ctx *ctx);
 /* Per-kernel device timing with HIP events on the context stream (0 = off). */
 xm_status xm_ctx_set_timing(xm_ctx *ctx, int enable);
 /* While timing is on, record only the launches of kernel `name` (NULL or "": all).
- * Each timed launch adds two event records to the stream (the 50M x 50M join
- * step, 45 launches: 7.94 ms timed, 7.56 ms untimed on MI355X), so a caller that
- * times one kernel inside a measured loop names it. */
+ * Each timed launch adds two event records to the stream: the 50M x 50M join step,
+ * when it made 45 launches, took 7.94 ms with all of them timed and 7.56 ms with
+ * none on MI355X.  A caller that times one kernel inside a measured loop names it. */
 xm_status xm_ctx_set_timing_only(xm_ctx *ctx, const char *name);
 /* Sum and count of recorded durations for kernel `name` since the last reset. */
 xm_status xm_ctx_kernel_time(xm_ctx *ctx, const char *name, double *total_ms, uint64_t *count);
