@@ -141,7 +141,8 @@ xm_status xm_memcpy_d2h(xm_ctx *ctx, void *dst, const void *src, size_t bytes);
  *      Sorter.cu:141-185) and Segment::Load's in-memory result (Segment.cpp:14-22).
  * Builds a device-resident, zone-sorted, zone-indexed catalog.  zone_height_deg
  * is the reference's zh_deg (BoostXmatch.cpp:158,271: default = radius).
- * The host variant copies the records to the device first. */
+ * The host variant copies the records to the device first.  n <= 2^32-8193
+ * (XM_EINVAL above). */
 xm_status xm_catalog_create(xm_ctx *ctx, const xm_obj *records, uint64_t n,
                             double zone_height_deg, xm_catalog **out);
 xm_status xm_catalog_create_device(xm_ctx *ctx, const xm_obj *d_records, uint64_t n,
@@ -149,7 +150,7 @@ xm_status xm_catalog_create_device(xm_ctx *ctx, const xm_obj *d_records, uint64_
 /* Many segments in one catalog, prepared together by one sort (the reference ran
  * one Sorter::Sort per segment, BoostXmatch.cpp:250-263,284-297).  Each segment
  * keeps its own zone height, zone table and rows; rows of segment s follow those
- * of segment s-1.  At most 65536 segments and 2^32-2 records in all.  The
+ * of segment s-1.  At most 65536 segments and 2^32-8193 records in all.  The
  * _device variant takes device record pointers; the other copies host records. */
 xm_status xm_catalog_create_segments(xm_ctx *ctx, const xm_segment *segs, uint64_t n_segs,
                                      xm_catalog **out);

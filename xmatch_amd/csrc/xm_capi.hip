@@ -282,7 +282,8 @@ uint64_t seg_table(const std::vector<const xm_obj*>& recs, const std::vector<uin
         d.zone_base = 0;
         d.zone_bits = d.ra_bits = d.idx_bits = 0;
         total += ns[s];
-        if (total >= 0xffffffffull) throw InvalidArg{"a catalog holds fewer than 2^32-1 records"};
+        if (total > SORT_MAX_KEYS)
+            throw InvalidArg{"a catalog holds at most 2^32-8193 records"};
     }
     return total;
 }
@@ -733,9 +734,9 @@ static xm_status catalog_create_impl(xm_ctx* ctx, const xm_segment* segs, uint64
             recs[s] = segs[s].records;
             ns[s] = segs[s].n;
             hs[s] = segs[s].zone_height_deg;
-            if (ns[s] >= 0xffffffffull) throw InvalidArg{"a segment holds fewer than 2^32-1 records"};
+            if (ns[s] > SORT_MAX_KEYS) throw InvalidArg{"a catalog holds at most 2^32-8193 records"};
             total += ns[s];
-            if (total >= 0xffffffffull) throw InvalidArg{"a catalog holds fewer than 2^32-1 records"};
+            if (total > SORT_MAX_KEYS) throw InvalidArg{"a catalog holds at most 2^32-8193 records"};
         }
         if (!on_device && total) {
             // one staging buffer for all segments, one copy each

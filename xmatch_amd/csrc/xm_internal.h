@@ -178,6 +178,9 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
                      int hi_bit, const uint32_t* digit_totals = nullptr);
 // digit positions of a 64-bit key in 8-bit digits
 constexpr int SORT_MAX_PASSES = 8;
+// Largest key count the radix sort takes (32-bit tile indices: the last 8192-key
+// tile must end below 2^32), and so the largest catalog.
+constexpr uint64_t SORT_MAX_KEYS = 0xffffffffull - 8192;
 
 // Adds one key's digits at positions lo_bit + 8p (p < n_pass) to the LDS
 // histogram h[p][256]; the active lanes of a wavefront call it together.  A digit

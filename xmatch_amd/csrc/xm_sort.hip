@@ -463,8 +463,8 @@ uint32_t* sort_error_word(xm_ctx* ctx) {
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                      int hi_bit, const uint32_t* digit_totals) {
     if (n <= 1 || hi_bit <= lo_bit) return;
-    // tile indices are 32-bit: the last tile's end must not pass 2^32
-    if (n > 0xffffffffull - RS_TILE)
+    static_assert(SORT_MAX_KEYS == 0xffffffffull - RS_TILE, "the key limit follows the tile size");
+    if (n > SORT_MAX_KEYS)
         throw InvalidArg{"radix sort supports up to 2^32 - " + std::to_string(RS_TILE + 1) + " keys"};
     if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
     // 8-bit digits starting at lo_bit (the last one may reach past hi_bit into bits
