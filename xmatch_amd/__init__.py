@@ -272,6 +272,12 @@ class Context:
         self._check(lib().xm_ctx_release_memory(self._h, ctypes.byref(b)))
         return b.value
 
+    def mem_info(self):
+        """(free, total) device bytes; blocks the context caches count as free."""
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().xm_ctx_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
     def set_param(self, name, value):
         self._check(lib().xm_ctx_set_param(self._h, name.encode(), int(value)))
 

@@ -70,13 +70,16 @@ int bit_length(uint32_t v) {
 void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     if (A->chunks || A->n == 0) return;
     const uint64_t entries = A->n_zoff - 1;  // zone entries; the last zoff entry is n
-    uint32_t* counts = static_cast<uint32_t*>(dalloc(ctx, A->n_zoff * sizeof(uint32_t)));
+    ScopedBlocks tmp(ctx);
+    uint32_t* counts = tmp.alloc<uint32_t>(A->n_zoff * sizeof(uint32_t));
     launch_fill_u32(ctx, counts + entries, 1, 0u);
     launch_chunk_counts(ctx, A->zoff, entries, counts);
     scan_exclusive_u32(ctx, counts, A->n_zoff);
     const uint64_t bound = A->n / CHUNK_ROWS + entries;
-    Chunk* chunks = static_cast<Chunk*>(dalloc(ctx, std::max<uint64_t>(bound, 1) * sizeof(Chunk)));
+    Chunk* chunks = tmp.alloc<Chunk>(std::max<uint64_t>(bound, 1) * sizeof(Chunk));
     launch_chunk_fill(ctx, A->zoff, counts, A->key, entries, chunks);
+    tmp.keep(chunks);
+    tmp.keep(counts);
     A->chunks = chunks;
     A->max_chunks = bound;
     A->chunk_start = counts;
@@ -322,15 +325,16 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     d.idx_bits = pl.idx_bits;
     d.zone_bits = pl.zone_bits;
     d.zone_base = 0;
-    uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-    uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    ScopedBlocks tmp(ctx);
+    uint64_t* words = tmp.alloc<uint64_t>(n * sizeof(uint64_t));
+    uint64_t* words_alt = tmp.alloc<uint64_t>(n * sizeof(uint64_t));
     // the sort's digit counts, taken while the keys are written: digit positions
     // from the row bits up over the ra bits and every zone bit (the sort may need
     // fewer once the occupied zone range is known)
     const int hist_pass = (pl.ra_bits + pl.zone_bits + 7) / 8;
     // scratch: zone range, bad count and dropped ra bits; layout statistics; digit counts
-    uint32_t* scratch = static_cast<uint32_t*>(dalloc(
-        ctx, (BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + size_t(hist_pass) * 256) * sizeof(uint32_t)));
+    uint32_t* scratch = tmp.alloc<uint32_t>(
+        (BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + size_t(hist_pass) * 256) * sizeof(uint32_t));
     launch_build_init(ctx, scratch, uint64_t(hist_pass) * 256);
     int32_t* small = reinterpret_cast<int32_t*>(scratch);
     uint32_t* stats = scratch + BUILD_SMALL_WORDS;
@@ -350,9 +354,6 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     d.ra_bits = pl.ra_bits - ra_drop;
     d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     if (nbad) {
-        dfree(ctx, scratch);
-        dfree(ctx, words);
-        dfree(ctx, words_alt);
         throw InvalidArg{std::to_string(nbad) +
                          " record(s) with NaN or out-of-range coordinates "
                          "(need 0 <= ra < 360, -90 <= dec <= 90)"};
@@ -365,12 +366,13 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
     radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals);
-    dfree(ctx, scratch);
+    tmp.free(scratch);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     launch_gather(ctx, d.recs, words, n, pl, ra_drop, zmin, static_cast<Row64*>(cat->row),
                   cat->zoff, cat->n_zoff, words_alt);
-    dfree(ctx, words);
+    tmp.free(words);
+    tmp.keep(words_alt);
     cat->key = words_alt;
 }
 
@@ -384,7 +386,8 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
     cat->seg_dev = static_cast<SegDesc*>(dalloc(ctx, S * sizeof(SegDesc)));
     XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice,
                           ctx->stream));
-    int32_t* range = static_cast<int32_t*>(dalloc(ctx, size_t(S) * 3 * sizeof(int32_t)));
+    ScopedBlocks tmp(ctx);
+    int32_t* range = tmp.alloc<int32_t>(size_t(S) * 3 * sizeof(int32_t));
     std::vector<int32_t> init(size_t(S) * 3);
     for (uint32_t s = 0; s < S; s++) {
         init[3 * s] = INT_MAX;
@@ -397,7 +400,7 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
     XM_HIP(hipMemcpyAsync(init.data(), range, init.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     sync(ctx);
-    dfree(ctx, range);
+    tmp.free(range);
     uint64_t nmax = 0;
     for (uint32_t s = 0; s < S; s++) {
         if (init[3 * s + 2])
@@ -431,19 +434,20 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
     zoff_layout(ctx, cat);
     XM_HIP(hipMemcpyAsync(cat->seg_dev, cat->seg.data(), S * sizeof(SegDesc), hipMemcpyHostToDevice,
                           ctx->stream));
-    uint64_t* words = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
-    uint64_t* words_alt = static_cast<uint64_t*>(dalloc(ctx, n * sizeof(uint64_t)));
+    uint64_t* words = tmp.alloc<uint64_t>(n * sizeof(uint64_t));
+    uint64_t* words_alt = tmp.alloc<uint64_t>(n * sizeof(uint64_t));
     // the sort's digit counts, taken while the words are written
     const int n_pass = (content + seg_bits - idx_bits + 7) / 8;
-    uint32_t* digit_totals = static_cast<uint32_t*>(dalloc(ctx, size_t(n_pass) * 256 * 4));
+    uint32_t* digit_totals = tmp.alloc<uint32_t>(size_t(n_pass) * 256 * 4);
     launch_fill_u32(ctx, digit_totals, uint64_t(n_pass) * 256, 0u);
     launch_seg_words(ctx, cat->seg_dev, S, n, seg_shift, words, idx_bits, n_pass, digit_totals);
     radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits, digit_totals);
-    dfree(ctx, digit_totals);
+    tmp.free(digit_totals);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row),
                       cat->zoff, cat->n_zoff, words_alt);
-    dfree(ctx, words);
+    tmp.free(words);
+    tmp.keep(words_alt);
     cat->key = words_alt;
     sync(ctx);  // the host segment table is rewritten next; its uploads must be done
 }

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -149,6 +150,51 @@ struct Chunk {
 void* dalloc(xm_ctx* ctx, size_t bytes);
 void dfree(xm_ctx* ctx, void* p);
 void release_cache(xm_ctx* ctx);
+
+// Device blocks a call takes for its own use.  They go back to the context's cache
+// when the object goes out of scope, also when an exception ends the call early
+// (kernels already enqueued on the stream still run first, as for any freed
+// block).  keep() hands a block on to an owner that outlives the call.
+class ScopedBlocks {
+  public:
+    explicit ScopedBlocks(xm_ctx* ctx) : ctx_(ctx) {}
+    ScopedBlocks(const ScopedBlocks&) = delete;
+    ScopedBlocks& operator=(const ScopedBlocks&) = delete;
+    ~ScopedBlocks() {
+        for (void* p : held_) {
+            try {
+                dfree(ctx_, p);
+            } catch (...) {
+            }
+        }
+    }
+    template <typename T>
+    T* alloc(size_t bytes) {
+        held_.reserve(held_.size() + 1);  // the push below cannot throw once dalloc has
+        void* p = dalloc(ctx_, bytes);
+        held_.push_back(p);
+        return static_cast<T*>(p);
+    }
+    void free(void* p) {
+        forget(p);
+        dfree(ctx_, p);
+    }
+    void keep(void* p) { forget(p); }
+
+  private:
+    void forget(void* p) {
+        for (size_t i = 0; i < held_.size(); i++) {
+            if (held_[i] == p) {
+                held_[i] = held_.back();
+                held_.pop_back();
+                return;
+            }
+        }
+        throw std::logic_error("a device block the call does not hold");
+    }
+    xm_ctx* ctx_;
+    std::vector<void*> held_;
+};
 
 // ---- launch bookkeeping: events around a kernel when timing is on
 void timing_begin(xm_ctx* ctx, const char* name, hipEvent_t* ev);

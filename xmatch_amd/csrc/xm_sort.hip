@@ -436,7 +436,8 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
         return;
     }
     if (nt > 0xffffffffull) throw InvalidArg{"scan too large"};
-    uint32_t* partial = static_cast<uint32_t*>(dalloc(ctx, nt * sizeof(uint32_t)));
+    ScopedBlocks tmp(ctx);
+    uint32_t* partial = tmp.alloc<uint32_t>(nt * sizeof(uint32_t));
     launch_timed(ctx, "scan_reduce", [&] {
         hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(nt)), dim3(SC_THREADS), 0, ctx->stream,
                            data, n, partial);
@@ -449,7 +450,6 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
         hipLaunchKernelGGL(k_scan_apply, dim3(uint32_t(nt)), dim3(SC_THREADS), 0, ctx->stream,
                            data, n, (const uint32_t*)partial);
     });
-    dfree(ctx, partial);
 }
 
 uint32_t* sort_error_word(xm_ctx* ctx) {
@@ -479,7 +479,8 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
     const size_t status_bytes = size_t(nblocks) * 256 * (narrow ? 4 : 8);
     const size_t ctr_words = 64;   // SORT_MAX_PASSES tile counters, padded
     const size_t digit_words = size_t(n_pass) * 256;
-    char* blk = static_cast<char*>(dalloc(ctx, status_bytes + (ctr_words + 2 * digit_words) * 4));
+    ScopedBlocks tmp(ctx);
+    char* blk = tmp.alloc<char>(status_bytes + (ctr_words + 2 * digit_words) * 4);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(blk + status_bytes);
     uint32_t* total = ctr + ctr_words;
     uint32_t* start = total + digit_words;
@@ -499,7 +500,6 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
         scatter_passes<uint32_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
     else
         scatter_passes<uint64_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
-    dfree(ctx, blk);
 }
 
 void sort_check(xm_ctx* ctx) {
