@@ -220,6 +220,43 @@ def test_spill_pool_takes_region_overflow(gpu_ctx):
     assert_same(got, O.match(A, B, r)[0])
 
 
+@pytest.mark.parametrize("case", ["r3deg_pole_cap", "r1deg_south", "r0.2deg_seam",
+                                  "r0.05arcsec_dense"])
+def test_large_and_small_radii_vs_oracle(gpu_ctx, case):
+    # Radii far from the arcsecond scale: degree radii give few, tall zones, ra
+    # windows that reach the whole circle near the pole (the reference's alpha
+    # sentinel) and cross the 0/360 seam; a 0.05" radius gives 1.3e7 zones (24 zone
+    # bits in the sort key).  The set must equal the oracle's.
+    rng = np.random.default_rng(sum(map(ord, case)))
+    if case == "r3deg_pole_cap":
+        r, n = 3.0, 1000
+        ra, dec = synth.uniform_cap(rng, n, 0.0, 85.0, 10.0)
+    elif case == "r1deg_south":
+        r, n = 1.0, 3000
+        ra, dec = synth.uniform_cap(rng, n, 180.0, -60.0, 10.0)
+    elif case == "r0.2deg_seam":
+        r, n = 0.2, 15000
+        ra, dec = synth.uniform_cap(rng, n, 359.5, 0.0, 10.0)
+    else:
+        r, n = 0.05 / 3600, 50000
+        ra, dec = synth.uniform_patch(rng, n, 30.0, 30.2, 10.0, 10.2)
+    a = synth.make_objs(np.arange(n), ra, dec)
+    if case == "r0.05arcsec_dense":
+        bra, bdec = synth.perturb(rng, ra, dec, 0.02)
+    else:
+        if case == "r3deg_pole_cap":
+            bra, bdec = synth.uniform_cap(rng, n, 0.0, 85.0, 10.0)
+        elif case == "r1deg_south":
+            bra, bdec = synth.uniform_cap(rng, n, 180.0, -60.0, 10.0)
+        else:
+            bra, bdec = synth.uniform_cap(rng, n, 359.5, 0.0, 10.0)
+    b = synth.make_objs(10**9 + np.arange(n), bra, bdec)
+    ref, _ = O.match(a, b, r)
+    assert len(ref[0]) > 10_000
+    got, _ = gpu_join(gpu_ctx, a, b, r)
+    assert_same(got, ref)
+
+
 @pytest.mark.parametrize("where", ["seam", "north_pole", "south_pole", "dense_cluster", "mid"])
 def test_edge_geometry_vs_allpairs(gpu_ctx, where):
     rng = np.random.default_rng(abs(hash(where)) % 2**32)

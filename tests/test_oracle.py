@@ -104,6 +104,25 @@ def _rand_objs(rng, n, ra0, ra1, dec0, dec1, id0=0):
     return synth.make_objs(id0 + np.arange(n), ra, dec)
 
 
+@pytest.mark.parametrize("r_deg,ra_c,dec_c,n", [(3.0, 0.0, 85.0, 1000), (1.0, 180.0, -60.0, 2000),
+                                                (0.2, 359.5, 0.0, 4000)])
+def test_join_degree_radii_equals_allpairs(r_deg, ra_c, dec_c, n):
+    # Degree radii: tall zones, ra windows reaching the whole circle near the pole
+    # (the alpha sentinel) and windows across the 0/360 seam; both oracle modes
+    # must equal the all-pairs set.
+    rng = np.random.default_rng(int(r_deg * 10))
+    ra, dec = synth.uniform_cap(rng, n, ra_c, dec_c, 10.0)
+    a = synth.make_objs(np.arange(n), ra, dec)
+    bra, bdec = synth.uniform_cap(rng, n, ra_c, dec_c, 10.0)
+    b = synth.make_objs(10**9 + np.arange(n), bra, bdec)
+    ref = O.brute(a, b, r_deg)
+    assert len(ref[0]) > 5000
+    for mode in (0, 1):
+        (ia, ib, d), _ = O.match(a, b, r_deg, mode=mode, threads=4)
+        assert np.array_equal(ia, ref[0]) and np.array_equal(ib, ref[1])
+        assert np.abs(d - ref[2]).max() <= 1e-9
+
+
 @pytest.mark.parametrize("where", ["equator", "seam", "north_pole", "south_pole", "mid"])
 def test_join_equals_allpairs(where):
     rng = np.random.default_rng(hash(where) % 2**32)
