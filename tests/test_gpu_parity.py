@@ -255,6 +255,9 @@ def test_large_and_small_radii_vs_oracle(gpu_ctx, case):
     assert len(ref[0]) > 10_000
     got, _ = gpu_join(gpu_ctx, a, b, r)
     assert_same(got, ref)
+    # the FP32 filter with FP64 recheck returns the same set at these radii too
+    got32, _ = gpu_join(gpu_ctx, a, b, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+    assert_same(got32, ref)
 
 
 @pytest.mark.parametrize("where", ["seam", "north_pole", "south_pole", "dense_cluster", "mid"])
