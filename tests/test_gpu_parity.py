@@ -298,6 +298,24 @@ def test_zone_height_below_radius(gpu_ctx):
     assert_same(got, O.brute(a, b, r, zone_height_deg=r / 3))
 
 
+@pytest.mark.parametrize("h_over_r", [1.7, 4.0, 30.0])
+def test_zone_height_above_radius(gpu_ctx, h_over_r):
+    # Zones taller than the radius (the reference's -z option above -r): one
+    # neighbour zone each side, and each zone holds many more rows per search.
+    rng = np.random.default_rng(int(h_over_r * 10))
+    ra, dec = synth.uniform_patch(rng, 20000, 30, 30.1, 10, 10.1)
+    a = synth.make_objs(np.arange(20000), ra, dec)
+    bra, bdec = synth.perturb(rng, ra, dec, 2.0)
+    b = synth.make_objs(10**9 + np.arange(20000), bra, bdec)
+    r = 3 / 3600
+    h = r * h_over_r
+    ref, _ = O.match(a, b, r, zone_height_deg=h)
+    assert len(ref[0]) > 10_000
+    got, _ = gpu_join(gpu_ctx, a, b, r, h=h)
+    assert_same(got, ref)
+    assert_same(got, O.brute(a, b, r, zone_height_deg=h))
+
+
 def test_boundary_pairs_match_oracle_exactly(gpu_ctx):
     # pairs at distance r(1 + eps) with eps down to 2^-50: rounding decides these,
     # so only an identical FP64 evaluation gives the oracle's set
