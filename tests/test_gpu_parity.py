@@ -360,6 +360,23 @@ def test_empty_and_single(gpu_ctx):
     assert abs(got[2][0] - 0.5) < 1e-6
 
 
+def test_ids_are_carried_bit_for_bit(gpu_ctx):
+    # Ids are opaque int64 payload (Obj.h:21-25): the extremes, negatives and
+    # duplicates come back exactly as given, paired as the positions say.
+    ids_a = np.array([-2**63, -1, 0, 2**63 - 1, 42, 42], dtype=np.int64)
+    ids_b = np.array([2**63 - 1, -2**63, 123456789012345678, -5, 42, 7], dtype=np.int64)
+    ra = np.array([0.0, 10.0, 20.0, 359.9999999, 100.0, 200.0])
+    dec = np.array([-90.0, 0.0, 45.0, 0.0, 89.9999, -30.0])
+    a = synth.make_objs(ids_a, ra, dec)
+    b = synth.make_objs(ids_b, ra, dec + 0.2 / 3600 * np.where(dec > 89, -1, 1))
+    r = 1 / 3600
+    got, _ = gpu_join(gpu_ctx, a, b, r)
+    ref, _ = O.match(a, b, r)
+    assert_same(got, ref)
+    pairs = sorted(zip(got[0].tolist(), got[1].tolist()))
+    assert pairs == sorted(zip(ids_a.tolist(), ids_b.tolist()))
+
+
 @pytest.mark.parametrize("bad", ["nan_ra", "nan_dec", "dec_high", "dec_low", "ra_360", "ra_neg"])
 def test_invalid_records_rejected(gpu_ctx, bad):
     a = synth.make_objs([1, 2, 3], [10.0, 20.0, 30.0], [0.0, 1.0, 2.0])
