@@ -461,7 +461,7 @@ def main():
                           "bytes_written": 1 << 30, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library drives one GPU and has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.config == "plane":

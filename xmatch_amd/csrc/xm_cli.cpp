@@ -44,7 +44,7 @@ struct Options {
     uint64_t maxout = 0;         // -m: first result buffer per job (0 = library default)
     double sr_arcsec = 5.0;      // -r
     double zh_arcsec = 0.0;      // -z (0 = radius)
-    uint32_t threads = 0;        // -t (GPUs; this build drives one)
+    uint32_t threads = 0;        // -t (GPUs; the library drives one)
     uint64_t num_obj = 0;        // -n (0 = whole file)
     uint32_t verbosity = 0;      // -v
     bool overwrite = false;      // -x
@@ -62,7 +62,7 @@ const char* USAGE =
     "                                results are never truncated\n"
     "  -r [ --radius ] arg (=5)      search radius in arcsec\n"
     "  -z [ --zoneheight ] arg (=0)  zone height in arcsec, defaults to radius\n"
-    "  -t [ --threads ] arg (=0)     number of GPUs (this build drives one)\n"
+    "  -t [ --threads ] arg (=0)     number of GPUs (the library drives one)\n"
     "  -n [ --numobj ] arg           number of objects per segment (default: whole file)\n"
     "  -v [ --verbose ] [=arg(=9)]   verbosity\n"
     "  -x [ --overwrite ]            overwrite output directory\n"
@@ -412,7 +412,7 @@ int main(int argc, char** argv) {
                   << "Zone height    :  " << o.zh_arcsec << "\n"
                   << "# of obj/seg   :  " << (o.num_obj ? std::to_string(o.num_obj) : "whole file")
                   << std::endl;
-        if (o.threads > 1) std::cerr << "-t " << o.threads << ": this build drives one GPU\n";
+        if (o.threads > 1) std::cerr << "-t " << o.threads << ": the library drives one GPU\n";
         if (nB < nA) std::cerr << "!! Larger file in memory?!" << std::endl;  // BoostXmatch.cpp:240
     }
     if (!prepare_outdir(o)) return 255;
