@@ -193,8 +193,8 @@ struct LbWord {
 // polling) turns a broken invariant into an error instead of a stalled kernel.
 constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 // earlier tiles' words read per round trip of the look-back (measured on the
-// 50M-key sort: 4 beats 1, 2 and 8; wider windows cost more traffic than the
-// round trips they save)
+// 50M-key sort: 4 beats 1, 2, 8 and 16; 3, 5 and 6 are within 1%; wider windows
+// cost more traffic than the round trips they save)
 constexpr int LB_WINDOW = 4;
 
 // total[pass][digit] += counts of every digit value over the block's keys (total
