@@ -370,9 +370,14 @@ class Context:
 
 
 class DeviceBuffer:
-    def __init__(self, ctx, ptr, nbytes, dtype, shape, owner=True):
+    """Device memory of a context.  An owning buffer returns its memory to the
+    context's cache on free() or when it is garbage-collected while the context is
+    open; a view keeps its parent alive."""
+
+    def __init__(self, ctx, ptr, nbytes, dtype, shape, owner=True, parent=None):
         self.ctx, self.ptr, self.nbytes, self.dtype, self.shape = ctx, ptr, nbytes, dtype, shape
         self.owner = owner
+        self._parent = parent
 
     def view(self, first, count):
         """Rows [first, first + count) of a 1-D buffer, without ownership."""
@@ -380,12 +385,18 @@ class DeviceBuffer:
         if first < 0 or count < 0 or (first + count) * item > self.nbytes:
             raise IndexError("view outside the buffer")
         return DeviceBuffer(self.ctx, ctypes.c_void_p((self.ptr.value or 0) + first * item),
-                            count * item, self.dtype, (int(count),), owner=False)
+                            count * item, self.dtype, (int(count),), owner=False, parent=self)
 
     def free(self):
-        if self.ptr and self.owner:
+        if self.ptr and self.owner and self.ctx.handle:
             self.ctx._check(lib().xm_device_free(self.ctx.handle, self.ptr))
         self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class SegmentTable:
@@ -403,6 +414,7 @@ class SegmentTable:
             if isinstance(r, DeviceBuffer) != self.on_device:
                 raise TypeError("segments must be all host arrays or all DeviceBuffers")
             if self.on_device:
+                self.keep.append(r)   # the table points into it
                 self.segs[i].records = r.ptr
                 self.segs[i].n = r.nbytes // OBJ_DTYPE.itemsize
             else:
