@@ -240,9 +240,11 @@ xm_status xm_server_submit(xm_server *srv, const xm_obj *records, uint64_t n, do
 xm_status xm_server_poll(xm_server *srv, uint64_t ticket, int32_t timeout_ms, uint64_t *n_matches);
 /* Copy a done request's pairs (query id, resident id, distance in arcsec) into
  * host arrays of length n_matches (any may be NULL) and its submit-to-done time,
- * then forget the request. */
+ * then forget the request.  A failed request is forgotten as well and its status
+ * returned: read its message with xm_server_error first. */
 xm_status xm_server_fetch(xm_server *srv, uint64_t ticket, int64_t *id_query, int64_t *id_catalog,
                           double *dist_arcsec, double *latency_ms);
+/* Message of a failed request (valid until the request is fetched). */
 const char *xm_server_error(xm_server *srv, uint64_t ticket);
 xm_status xm_server_get_stats(xm_server *srv, xm_server_stats *stats);
 /* Finishes the queued requests, stops the worker and frees the server. */

@@ -591,7 +591,10 @@ class Server:
         if st == XM_EAGAIN:
             return None
         if st != XM_OK:
-            raise XmatchError(st, lib().xm_server_error(self._h, ticket).decode())
+            msg = lib().xm_server_error(self._h, ticket).decode()
+            # a failed request is released by fetching it
+            lib().xm_server_fetch(self._h, ticket, None, None, None, None)
+            raise XmatchError(st, msg)
         return n.value
 
     def result(self, ticket, timeout_ms=-1):

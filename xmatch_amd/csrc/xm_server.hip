@@ -310,8 +310,8 @@ xm_status xm_server_fetch(xm_server* srv, uint64_t ticket, int64_t* id_query, in
         auto it = srv->live.find(ticket);
         if (it == srv->live.end() || !it->second->done) return XM_EINVAL;
         r = it->second;
+        srv->live.erase(it);  // a failed request is forgotten too
         if (r->status != XM_OK) return r->status;
-        srv->live.erase(it);
     }
     const size_t n = r->id_q.size();
     if (id_query && n) std::memcpy(id_query, r->id_q.data(), n * 8);
