@@ -1134,6 +1134,7 @@ xm_status xm_result_copy(xm_ctx* ctx, const xm_result* res, int64_t* id_a, int64
     if (!ctx || !res) return XM_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     return guarded(ctx, [&] {
+        if (res->ctx != ctx) throw InvalidArg{"result belongs to another context"};
         const size_t bytes = res->n * 8;
         if (!bytes) return;
         if (id_a) XM_HIP(hipMemcpyAsync(id_a, res->id_a, bytes, hipMemcpyDeviceToHost, ctx->stream));
