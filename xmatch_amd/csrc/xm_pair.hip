@@ -302,7 +302,8 @@ __device__ inline uint32_t xcd_swizzle(uint32_t b, uint32_t nb) {
 // same two unit vectors the pair kernel tested, so the bits agree.  chord^2 turns
 // into 2 asin(sqrt(d2)/2) in arcsec.  The part prefix is searched in LDS.
 constexpr uint32_t COMPACT_ITEMS = 4;  // measured on 30M matches: 1 -> 0.329 ms, 4 -> 0.267, 16 -> 0.275
-__global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
+constexpr uint32_t COMPACT_TPB = 1024; // 30M matches: 256 threads 0.267 ms, 1024 0.257
+__global__ __launch_bounds__(COMPACT_TPB) void k_compact(uint32_t n_parts,
                                                  const uint64_t* __restrict__ prefix,
                                                  const uint64_t* __restrict__ base,
                                                  uint64_t total,
@@ -320,15 +321,15 @@ __global__ __launch_bounds__(256) void k_compact(uint32_t n_parts,
                                                  uint32_t* __restrict__ job) {
     __shared__ uint64_t s_pre[MAX_REGIONS + SPILL_POOLS];
     __shared__ uint64_t s_base[MAX_REGIONS + SPILL_POOLS];
-    for (uint32_t r = threadIdx.x; r < n_parts; r += 256) {
+    for (uint32_t r = threadIdx.x; r < n_parts; r += COMPACT_TPB) {
         s_pre[r] = prefix[r];
         s_base[r] = base[r];
     }
     __syncthreads();
     // COMPACT_ITEMS outputs per thread, so each block's load of the part tables
-    // serves 256 * COMPACT_ITEMS outputs
+    // serves COMPACT_TPB * COMPACT_ITEMS outputs
     for (uint32_t k = 0; k < COMPACT_ITEMS; k++) {
-        const uint64_t i = (uint64_t(blockIdx.x) * COMPACT_ITEMS + k) * 256 + threadIdx.x;
+        const uint64_t i = (uint64_t(blockIdx.x) * COMPACT_ITEMS + k) * COMPACT_TPB + threadIdx.x;
         if (i >= total) return;
         uint32_t lo = 0, hi = n_parts - 1;  // last part with prefix <= i
         while (lo < hi) {
@@ -751,8 +752,9 @@ void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, ui
     if (total == 0) return;
     launch_timed(ctx, "compact", [&] {
         hipLaunchKernelGGL(k_compact,
-                           dim3(uint32_t((total + 256 * COMPACT_ITEMS - 1) / (256 * COMPACT_ITEMS))),
-                           dim3(256), 0,
+                           dim3(uint32_t((total + COMPACT_TPB * COMPACT_ITEMS - 1) /
+                                         (COMPACT_TPB * COMPACT_ITEMS))),
+                           dim3(COMPACT_TPB), 0,
                            ctx->stream, ro.n_regions + SPILL_POOLS, prefix, ro.base, total,
                            (const int64_t*)(ro.wide ? ro.id_a : nullptr),
                            (const int64_t*)ro.id_b, (const double*)ro.d2,
