@@ -345,6 +345,12 @@ __device__ inline void store_row_nt(Row64* p, const Row64& r) {
     __builtin_nontemporal_store(b, reinterpret_cast<d2*>(p) + 1);
 }
 
+// k_gather takes one row per thread in blocks of GATHER_TPB: 50M random-order
+// records, 256 threads 1.61 ms, 512 1.60 ms, 1024 1.59 ms.  k_gather_seg keeps
+// blocks of TPB: the 10,000-segment batch's gather took 49.5 ms with 256 threads and
+// 50.1 ms with 1024.
+constexpr int GATHER_TPB = 1024;
+
 __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
                                                     const uint64_t* __restrict__ words, uint32_t n,
                                                     Row64* __restrict__ rows,
@@ -380,10 +386,6 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
     __builtin_nontemporal_store(
         (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb)), &keys[i]);
 }
-
-// One row per thread in blocks of GATHER_TPB: 50M random-order records, 256 threads
-// 1.61 ms, 512 1.60 ms, 1024 1.59 ms.
-constexpr int GATHER_TPB = 1024;
 
 // A sort whose look-back failed (sort_err set) leaves words that may index past
 // the records and the zone table: the gathers then write nothing (the catalog call
