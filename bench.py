@@ -1,18 +1,29 @@
 """Benchmark of the zone crossmatch on one MI355X.
 
-Workload (default): full-sky plain FP64 join, 50M x 50M, radius = zone height =
-2 arcsec.  A: uniform on the sphere; B: 60% perturbed copies of A (0.5" Gaussian
-per tangent-plane axis) + 40% fresh uniform.  Synthetic data, seeded PCG64.
+Headline workload (default): the precision form of the join, 200M x 200M all-sky
+with a galactic-plane-like density, radius = zone height = 1 arcsec.  Source
+density is proportional to 1 + 20 exp(-b^2 / (2 (10 deg)^2)), b = latitude
+relative to the great circle whose pole is at (ra, dec) = (0, 30) deg, 60 deg from
+the celestial pole.  B: 50% copies of A sources moved by a 0.3" Gaussian per
+tangent-plane axis, 50% fresh sources of the same density.  Both catalogs are
+generated on the device (Philox-4x32-10).
 
-One step = the hot path over one batch: prepare catalog A (keys, radix sort, zone
-index, unit vectors), prepare catalog B, join A x B into device-resident
-(idA, idB, dist) triples.  Input records are resident in HBM before timing.
+One step = the hot path over one batch, records resident in HBM: prepare catalog
+A (keys, radix sort, zone index, unit vectors), prepare catalog B, join A x B into
+device-resident (idA, idB, dist) triples.  The step runs in both precision forms
+on the same records:
+  fp64_plain                FP64 rows, the FP64 chord on every candidate;
+  fp32_filter_fp64_recheck  the catalogs also carry FP32 rows, written by the same
+                            gather pass as the FP64 rows; the FP32 chord filters
+                            with a guard band, every kept pair is rechecked in FP64.
+Both catalogs are rebuilt in every step of both forms: nothing carries over.
 
-Metric: pair-tests/s = W_cand / step time, W_cand = pairs reaching the exact chord
-test under the reference's zone/alpha windows (host-computed from the sorted zone
-tables).  matches/s, join-only pair-tests/s and the radix-sort GB/s are reported
-beside it.  Per-kernel durations are measured with HIP events on the stream the
-kernels run on.
+Metric: pair-tests/s = W_cand / step time.  W_cand = pairs reaching the exact chord
+test under the reference's zone and alpha windows, computed on the host from the
+sorted zone tables.  value = the FP32-filter form's rate; the FP64 form's rate, the
+ratio, the two match sets' digests, the band hits and the largest FP32 chord error
+go beside it, with matches/s and the join-only rates.  Per-kernel durations are
+measured with HIP events on the stream the kernels run on.
 """
 import argparse
 import json
@@ -29,9 +40,12 @@ import xmatch_amd as xm  # noqa: E402
 from xmatch_amd import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+# a pair test in difference form: 3 sub, 1 mul, 2 FMA (= 4 FLOP) -> 8 FLOP
+FLOP_PER_PAIR_TEST = 8.0
 # untimed steps with events around every launch, for the per-kernel breakdown
 BREAKDOWN_STEPS = 3
-FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 # algorithmic HBM bytes per source for the memory-bound kernels of a catalog build
 BYTES_PER_SOURCE = {
@@ -39,10 +53,17 @@ BYTES_PER_SOURCE = {
     "radix_hist": 8.0,       # read packed word (once per sort, every digit)
     "make_keys": 32.0,       # read 24-B record, write packed word 8
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
+    "gather_fp32": 88.0,     # the same + the 16-B FP32 row
 }
 KERNELS = ["layout", "make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
            "scan_top", "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
-           "pair_fp64", "region_scan", "compact"]
+           "pair_fp64", "pair_fp32", "to_row32", "region_scan", "compact", "job_prep"]
+
+# rocprofv3 names of kernels whose timing name differs
+PROFILE_NAMES = {"pair_fp32": "pair<true>", "pair_fp64": "pair<false>"}
+
+PLANE_SEED = 2003  # device generator key of the headline catalogs
+MODES = (("fp64_plain", xm.FP64_PLAIN), ("fp32_filter_fp64_recheck", xm.FP32_FILTER_FP64_RECHECK))
 
 
 def log(msg):
@@ -50,8 +71,10 @@ def log(msg):
 
 
 def measured_traffic(kernel, profile="full_sky_50m_fp64.md"):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (already scaled by the copy calibration), or None."""
+    """HBM bytes per launch of `kernel` (a rocprofv3 kernel name without its
+    template arguments, less the leading k_) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy calibration), or
+    None."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
@@ -69,22 +92,53 @@ def measured_traffic(kernel, profile="full_sky_50m_fp64.md"):
     return None
 
 
-def cpu_baseline(A, B, r, dec_band, threads):
-    """The CPU oracle (FP64 restatement of the reference zone join, dense zone-pair
-    tests with the reference's box prefilter) on a bounded sample: both catalogs
-    restricted to |dec| < dec_band (A) / dec_band + 2r (B).  Times segment
-    preparation + zone index + join, like one GPU step."""
+def host_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on, capped
+    by OMP_NUM_THREADS when it is set (the GPU box allots 16 host threads per GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        return min(avail, int(cap))
+    return avail
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline_band(A, B, r, threads, target_s):
+    """The CPU oracle (FP64 restatement of the reference zone join: its dense
+    zone-pair loop with the box and alpha prefilters, sort + zone index + join) on
+    a declination band |dec| < w of both catalogs (B widened by 2r), w sized from a
+    short calibration run so that the timed run takes about target_s seconds."""
     from oracle import oracle as O
-    sa = A[np.abs(A["dec"]) < dec_band]
-    sb = B[np.abs(B["dec"]) < dec_band + 2 * r]
-    t0 = time.perf_counter()
-    (ia, _, _), cnt = O.match(sa, sb, r, mode=0, threads=threads)
-    t = time.perf_counter() - t0
+
+    def run(w):
+        sa = A[np.abs(A["dec"]) < w]
+        sb = B[np.abs(B["dec"]) < w + 2 * r]
+        t0 = time.perf_counter()
+        (ia, _, _), cnt = O.match(sa, sb, r, mode=0, threads=threads)
+        return time.perf_counter() - t0, sa, sb, ia, cnt
+
+    w = 0.25
+    t, *_ = run(w)
+    w = float(min(60.0, max(0.05, w * target_s / max(t, 1e-3))))
+    t, sa, sb, ia, cnt = run(w)
     return {
         "value": cnt["w_cand"] / t, "unit": "pair-tests/s", "cores": threads, "kind": "port",
-        "sample": "A and B restricted to |dec| < %.1f deg (%d x %d sources, %.1f%% of A), "
-                  "sort + zone index + dense zone join, %.1f s" %
-                  (dec_band, len(sa), len(sb), 100.0 * len(sa) / len(A), t),
+        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+        "sample": "A and B restricted to |dec| < %.3f deg (B widened by 2r): %d x %d sources, "
+                  "%.2f%% of A; sort + zone index + the reference's dense zone join, %.1f s on "
+                  "%d threads" % (w, len(sa), len(sb), 100.0 * len(sa) / len(A), t, threads),
         "seconds": t, "w_cand": cnt["w_cand"], "w_ref": cnt["w_ref"], "matches": len(ia),
     }
 
@@ -99,60 +153,289 @@ def pair_digest(ida, idb, dist):
             "dist_bits": int(np.sum(dist.view(np.uint64), dtype=np.uint64))}
 
 
-def run_plane(args):
-    """Precision form: 200M x 200M with a galactic-plane-like density, r = h = 1",
-    FP64 plain against FP32 filter + FP64 recheck on the same catalogs."""
-    n = args.n if args.n != 50_000_000 else 200_000_000
+def kernel_table(ctx, steps):
+    out = {}
+    for k in KERNELS:
+        tot, cnt = ctx.kernel_time(k)
+        if cnt:
+            out[k] = {"total_ms": tot / steps, "launches": cnt // steps, "avg_ms": tot / cnt}
+    return out
+
+
+def timed_steps(ctx, step, warmup, steps, only=None):
+    """warmup untimed steps, BREAKDOWN_STEPS with events around every launch, then
+    `steps` timed steps (events around kernel `only` alone, None: none) bracketed by
+    device synchronisation.  Returns (ms per step, breakdown table, (total ms,
+    launches) of `only` over the timed steps, the last step's stats)."""
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    for _ in range(BREAKDOWN_STEPS):
+        step()
+    ctx.set_timing(False)
+    table = kernel_table(ctx, BREAKDOWN_STEPS)
+    if only is None:
+        only = max(table.items(), key=lambda kv: kv[1]["total_ms"])[0]
+    ctx.reset_timing()
+    ctx.set_timing(True, only=only)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(steps):
+        last = step()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    dom = ctx.kernel_time(only)
+    ctx.reset_timing()
+    return 1e3 * elapsed / steps, table, only, dom, last
+
+
+def make_plane(ctx, n, seed):
+    plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
+    dA = ctx.synth_fresh(plane, n, seed, stream=0, id0=0)
+    dB = ctx.synth_copies(dA, 0.5, 0.3, plane, n, seed, stream=1)
+    return dA, dB
+
+
+def join_step(ctx, dA, dB, r, prec):
+    """One step: both catalogs built from the resident records, then the join."""
+    def step():
+        ca = ctx.catalog(dA, r)
+        cb = ctx.catalog(dB, r)
+        res = ctx.match(ca, cb, r, precision=prec)
+        out = (res.count, res.stats.as_dict())
+        res.close(); ca.close(); cb.close()
+        return out
+    return step
+
+
+def pair_roofline(mode, avg_ms, n_a, n_b, matches, w_tile):
+    """Bytes roofline of the pair kernel: each catalog row read once (its key and
+    the rows the form reads) and each match written once as a 24-B (idA, idB,
+    chord^2) slot, over the kernel's average launch; and the VALU fraction, W_tile
+    tests of FLOP_PER_PAIR_TEST against the vector peak of the form's precision."""
+    if mode == "fp64_plain":
+        per_row, per_match, peak_tf = 8 + 32, 24, FP64_VECTOR_PEAK_TFLOPS
+        a_bytes, b_bytes = n_a * per_row, n_b * per_row
+    else:
+        # FP32 rows for the filter, plus the FP64 row of each matched B row for the
+        # recheck and the distance (every A row of a chunk with a kept pair: all of them)
+        per_match, peak_tf = 24 + 32, FP32_VECTOR_PEAK_TFLOPS
+        a_bytes, b_bytes = n_a * (8 + 16 + 32), n_b * (8 + 16)
+    alg = a_bytes + b_bytes + matches * per_match
+    ach = alg / (avg_ms / 1e3) / 1e9
+    valu = w_tile * FLOP_PER_PAIR_TEST / (avg_ms / 1e3) / 1e12
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg,
+            "avg_launch_ms": avg_ms,
+            "valu": {"achieved_tflops": valu, "peak_tflops": peak_tf, "frac": valu / peak_tf,
+                     "flop_per_test": FLOP_PER_PAIR_TEST, "w_tile": w_tile}}
+
+
+def run_full_sky_device(ctx, n, warmup, steps):
+    """The full-sky plain FP64 join, 50M x 50M, r = h = 2": A uniform on the
+    sphere, B 60% copies (0.5" per axis) + 40% fresh, generated on the device; one
+    step = prepare A + prepare B + join."""
+    r = 2.0 / 3600.0
+    sphere = xm.shape(xm.SHAPE_SPHERE)
+    dA = ctx.synth_fresh(sphere, n, 2002, stream=0, id0=0)
+    dB = ctx.synth_copies(dA, 0.6, 0.5, sphere, n, 2002, stream=1)
+    ms, table, dom, (dtot, dcnt), (nm, st) = timed_steps(
+        ctx, join_step(ctx, dA, dB, r, xm.FP64_PLAIN), warmup, steps)
+    ca, cb = ctx.catalog(dA, r), ctx.catalog(dB, r)
+    w = ctx.work_counts(ca, cb, r)
+    ca.close(); cb.close(); dA.free(); dB.free()
+    ctx.release_memory()
+    return {"workload": "full-sky plain FP64 zone join, %d x %d, r = h = 2 arcsec (device-"
+                        "generated: A uniform on the sphere, B 60%% copies at 0.5 arcsec + 40%% "
+                        "fresh)" % (n, n),
+            "value": w["w_cand"] / (ms / 1e3), "unit": "pair-tests/s", "ms_per_step": ms,
+            "steps": steps, "matches": nm, "matches_per_s": nm / (ms / 1e3),
+            "join_ms": st["join_ms"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
+            "w_ref": w["w_ref"], "dominant_kernel": dom, "dominant_avg_ms": dtot / dcnt,
+            "kernels": table}
+
+
+def run_headline(args):
+    """The precision form (default): 200M x 200M galactic-plane density at 1",
+    each step building both catalogs and joining, FP64 plain and FP32 filter +
+    FP64 recheck on the same records."""
+    n = args.n or 200_000_000
     r = 1.0 / 3600.0
     ctx = xm.Context(0)
-    plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
+    if args.chunk_rows:
+        ctx.set_param("chunk_rows", args.chunk_rows)
+    if args.autotune:
+        ctx.set_param("autotune", 1)
     t0 = time.time()
-    dA = ctx.synth_fresh(plane, n, args.seed * 1000 + 3, stream=0, id0=0)
-    dB = ctx.synth_copies(dA, 0.5, 0.3, plane, n, args.seed * 1000 + 3, stream=1)
-    log("generated %d x %d plane-density sources on the device in %.2f s" % (n, n, time.time() - t0))
-    ca = ctx.catalog(dA, r)
-    cb = ctx.catalog(dB, r)
-    out = {"metric": "pair-tests/s", "unit": "pair-tests/s", "n_gpus": 1, "data": "synthetic",
-           "config": {"workload": "galactic-plane density zone join, %d x %d, r = h = 1 arcsec"
-                      % (n, n), "n_a": n, "n_b": n, "radius_arcsec": 1.0,
-                      "density": "1 + 20 exp(-b^2 / (2 (10 deg)^2)), pole (0, 30)",
-                      "b": "50% copies (0.3 arcsec) + 50% fresh"}}
+    dA, dB = make_plane(ctx, n, PLANE_SEED)
+    log("generated %d x %d plane-density sources on the device in %.2f s"
+        % (n, n, time.time() - t0))
+    # work counts from the sorted zone tables (host), one build; with the tuner on,
+    # one join first, so that the counts follow the tuned shape
+    ca, cb = ctx.catalog(dA, r), ctx.catalog(dB, r)
+    if args.autotune:
+        ctx.match(ca, cb, r).close()
     w = ctx.work_counts(ca, cb, r)
-    out.update(w)
+    ca.close(); cb.close()
+    log("work counts: %s" % w)
     modes = {}
     digests = {}
-    for name, prec in (("fp64_plain", xm.FP64_PLAIN), ("fp32_filter_fp64_recheck",
-                                                       xm.FP32_FILTER_FP64_RECHECK)):
-        for _ in range(args.warmup):
-            ctx.match(ca, cb, r, precision=prec).close()
-        ctx.synchronize()
-        ms = []
-        for _ in range(args.steps):
-            t = time.perf_counter()
-            res = ctx.match(ca, cb, r, precision=prec)
-            ms.append(1e3 * (time.perf_counter() - t))
-            st = res.stats.as_dict()
-            last = res
-            if _ < args.steps - 1:
-                res.close()
-        ida, idb, dist = last.to_host(sort=False)
-        last.close()
-        digests[name] = pair_digest(ida, idb, dist)
-        del ida, idb, dist
-        join_ms = float(np.median(ms))
-        modes[name] = {"join_ms": join_ms, "device_join_ms": st["join_ms"],
-                       "pair_tests_per_s": w["w_cand"] / (join_ms / 1e3),
-                       "matches": st["n_matches"], "band_hits": st["band_hits"],
-                       "max_dist_err_arcsec": st["max_dist_err_arcsec"]}
-        log("%s: %.2f ms, %d matches" % (name, join_ms, st["n_matches"]))
-    out["modes"] = modes
-    out["identical_match_set"] = digests["fp64_plain"] == digests["fp32_filter_fp64_recheck"]
-    out["digest"] = digests["fp64_plain"]
-    out["fp32_over_fp64"] = (modes["fp64_plain"]["join_ms"] /
-                             modes["fp32_filter_fp64_recheck"]["join_ms"])
-    ca.close(); cb.close(); dA.free(); dB.free()
+    dom = None
+    # the FP32 form first: its dominant kernel is the one whose launches the timed
+    # steps of both forms carry events around
+    for name, prec in MODES[::-1]:
+        ctx.set_param("fp32_rows", 1 if prec == xm.FP32_FILTER_FP64_RECHECK else 0)
+        step = join_step(ctx, dA, dB, r, prec)
+        ms, table, only, (dtot, dcnt), (nm, st) = timed_steps(ctx, step, args.warmup,
+                                                              args.steps, only=dom)
+        dom = only
+        pk = "pair_fp32" if prec == xm.FP32_FILTER_FP64_RECHECK else "pair_fp64"
+        # one more step, untimed, whose pairs are kept for the digest
+        ca, cb = ctx.catalog(dA, r), ctx.catalog(dB, r)
+        res = ctx.match(ca, cb, r, precision=prec)
+        digests[name] = pair_digest(*res.to_host(sort=False))
+        res.close(); ca.close(); cb.close()
+        ctx.release_memory()
+        modes[name] = {
+            "ms_per_step": ms, "pair_tests_per_s": w["w_cand"] / (ms / 1e3),
+            "matches": nm, "matches_per_s": nm / (ms / 1e3),
+            "join_ms": st["join_ms"], "join_pair_tests_per_s": w["w_cand"] / (st["join_ms"] / 1e3),
+            "band_hits": st["band_hits"], "max_dist_err_arcsec": st["max_dist_err_arcsec"],
+            "n_retries": st["n_retries"],
+            "dominant_kernel_avg_ms": dtot / dcnt, "dominant_kernel_launches": dcnt,
+            "kernels": table,
+            "pair_roofline": pair_roofline(name, table[pk]["avg_ms"], n, n, nm, w["w_tile"]),
+        }
+        log("%s: %.2f ms per step (join %.2f ms), %d matches"
+            % (name, ms, st["join_ms"], nm))
+    ctx.set_param("fp32_rows", 0)
+    f32, f64 = modes["fp32_filter_fp64_recheck"], modes["fp64_plain"]
+    # the dominant kernel of the FP32 form: its algorithmic bytes per launch (the
+    # gather writes the FP32 rows too) over its average launch in the timed steps
+    if dom == "gather":
+        alg = BYTES_PER_SOURCE["gather_fp32"] * n
+    elif dom in BYTES_PER_SOURCE:
+        alg = BYTES_PER_SOURCE[dom] * n
+    elif dom == "pair_fp32":
+        alg = f32["pair_roofline"]["algorithmic_bytes_per_launch"]
+    else:
+        alg = None
+    dom_avg = f32["dominant_kernel_avg_ms"]
+    roof = {"kernel": dom, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "avg_launch_ms": dom_avg, "launches_timed": f32["dominant_kernel_launches"],
+            "traffic": measured_traffic(PROFILE_NAMES.get(dom, dom), "plane_200m_precision.md"),
+            "algorithmic_bytes_per_launch": alg, "achieved": None, "frac": None}
+    if alg:
+        roof["achieved"] = alg / (dom_avg / 1e3) / 1e9
+        roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    line = {
+        "metric": "pair-tests/s", "value": f32["pair_tests_per_s"], "unit": "pair-tests/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 filter + f64 recheck",
+        "data": "synthetic (generated on the device, Philox-4x32-10)",
+        "config": {"workload": "galactic-plane density zone join, %d x %d, r = h = 1 arcsec, "
+                               "FP32 filter + FP64 recheck against FP64 plain; a step builds "
+                               "both catalogs and joins" % (n, n),
+                   "n_a": n, "n_b": n, "radius_arcsec": 1.0, "seed": PLANE_SEED,
+                   "density": "1 + 20 exp(-b^2 / (2 (10 deg)^2)), pole (ra, dec) = (0, 30) deg",
+                   "b": "50% copies (0.3 arcsec per axis) + 50% fresh"},
+        "matches": f32["matches"], "matches_per_s": f32["matches_per_s"],
+        "fp64_plain_value": f64["pair_tests_per_s"],
+        "fp32_over_fp64": f32["pair_tests_per_s"] / f64["pair_tests_per_s"],
+        "fp32_over_fp64_join_only": f64["join_ms"] / f32["join_ms"],
+        "identical_match_set": digests["fp64_plain"] == digests["fp32_filter_fp64_recheck"],
+        "digest": digests["fp64_plain"],
+        "band_hits": f32["band_hits"], "max_dist_err_arcsec": f32["max_dist_err_arcsec"],
+        "w_ref": w["w_ref"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
+        "modes": modes, "roofline": roof,
+        "pair_shape": {"chunk_rows": ctx.get_param("chunk_rows"),
+                       "dense_min": ctx.get_param("dense_min"),
+                       "autotune": bool(args.autotune), "tune_table": ctx.tune_table()},
+        "kernels_from": "%d untimed steps with events around every launch; the timed steps "
+                        "carry events around the dominant kernel only" % BREAKDOWN_STEPS,
+    }
+    if not args.no_full_sky:
+        log("full-sky 50M x 50M FP64 ...")
+        line["full_sky_50m"] = run_full_sky_device(ctx, 50_000_000, 2, 10)
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or host_threads()
+        log("cpu baseline on %d threads ..." % threads)
+        A, B = ctx.download(dA), ctx.download(dB)
+        line["cpu_baseline"] = cpu_baseline_band(A, B, r, threads, args.cpu_seconds)
+        del A, B
+    else:
+        line["cpu_baseline"] = None
+    dA.free(); dB.free()
     ctx.close()
-    print(json.dumps(out), flush=True)
+    print(json.dumps(line), flush=True)
+
+
+def run_full_sky(args):
+    """The full-sky plain FP64 join, 50M x 50M, r = h = 2" (host-generated, seeded
+    PCG64), one step = prepare A + prepare B + join; the CPU oracle beside it on
+    |dec| < cpu_dec_band."""
+    from oracle import oracle as O  # noqa: F401  (the cpu_baseline leg)
+    n = args.n or 50_000_000
+    t0 = time.time()
+    A, B, r = synth.config_full_sky(seed=args.seed, n_a=n, n_b=n)
+    log("generated %d x %d sources in %.1f s" % (len(A), len(B), time.time() - t0))
+    if args.input_order == "dec":
+        # spatially ordered input, as survey catalogs are often stored (a diagnostic
+        # form; the workload keeps the random order)
+        A = A[np.argsort(A["dec"], kind="stable")]
+        B = B[np.argsort(B["dec"], kind="stable")]
+    ctx = xm.Context(0)
+    dA = ctx.to_device(A)
+    dB = ctx.to_device(B)
+    ms, table, dom, (dtot, dcnt), (nm, st) = timed_steps(
+        ctx, join_step(ctx, dA, dB, r, xm.FP64_PLAIN), args.warmup, args.steps)
+    ca = ctx.catalog(dA, r)
+    cb = ctx.catalog(dB, r)
+    w = ctx.work_counts(ca, cb, r)
+    ca.close(); cb.close()
+    dom_avg = dtot / dcnt
+    per_launch = BYTES_PER_SOURCE.get(dom, 0.0) * n
+    ach = per_launch / (dom_avg / 1e3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+            "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": dom_avg,
+            "launches_timed": dcnt}
+    sort_ms = sum(table.get(k, {}).get("total_ms", 0.0)
+                  for k in ("radix_hist", "radix_offsets", "radix_scatter"))
+    passes = table.get("radix_scatter", {}).get("launches", 0) / 2
+    sort_bytes = 2 * n * (8 + passes * 16)
+    line = {
+        "metric": "pair-tests/s", "value": w["w_cand"] / (ms / 1e3), "unit": "pair-tests/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "full-sky plain FP64 zone join, %d x %d, r = h = 2 arcsec" % (n, n),
+                   "n_a": n, "n_b": n, "radius_arcsec": 2.0, "seed": args.seed,
+                   "input_order": args.input_order},
+        "matches": nm, "matches_per_s": nm / (ms / 1e3),
+        "join_ms": st["join_ms"], "join_pair_tests_per_s": w["w_cand"] / (st["join_ms"] / 1e3),
+        "w_ref": w["w_ref"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
+        "sort_ms_per_step": sort_ms, "sort_passes_per_catalog": passes,
+        "sort_gb_per_s": sort_bytes / (sort_ms / 1e3) / 1e9 if sort_ms else None,
+        "kernels": table, "kernels_from": "%d untimed steps with events around every launch"
+                                          % BREAKDOWN_STEPS,
+        "roofline": roof,
+        "pair_roofline": pair_roofline("fp64_plain", table["pair_fp64"]["avg_ms"], n, n, nm,
+                                       w["w_tile"]),
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or host_threads()
+        log("cpu baseline on %d threads ..." % threads)
+        line["cpu_baseline"] = cpu_baseline_band(A, B, r, threads, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(line), flush=True)
 
 
 def run_batch(args):
@@ -339,7 +622,7 @@ def run_serve(args):
     answers requests arriving as a Poisson process; latency = submit -> done,
     measured by the server; p50 / p99 per arrival rate."""
     import threading
-    n_res = args.n if args.n != 50_000_000 else 500_000_000
+    n_res = args.n or 500_000_000
     ctx = xm.Context(0)
     plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
     t0 = time.time()
@@ -422,12 +705,13 @@ def run_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="full_sky",
-                    choices=["full_sky", "plane", "batch", "serve"],
-                    help="full_sky: the 50M x 50M FP64 join (default); plane: the 200M x 200M "
-                         "precision-form comparison; batch: many independent joins, batched "
-                         "against a loop of one-job calls; serve: a resident catalog "
-                         "answering Poisson request streams")
+    ap.add_argument("--config", default="plane",
+                    choices=["plane", "full_sky", "batch", "serve"],
+                    help="plane: the 200M x 200M precision-form join, FP32 filter against "
+                         "FP64 plain (default, the headline); full_sky: the 50M x 50M FP64 "
+                         "join; batch: many independent joins, batched against a loop of "
+                         "one-job calls; serve: a resident catalog answering Poisson request "
+                         "streams")
     ap.add_argument("--rates", default="50,200,800", help="serve config: arrival rates, q/s")
     ap.add_argument("--duration", type=float, default=10.0,
                     help="serve config: seconds of arrivals per rate")
@@ -439,17 +723,31 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=50_000_000, help="sources per catalog")
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--n", type=int, default=0,
+                    help="sources per catalog (0: the config's own: plane 2e8, full_sky 5e7, "
+                         "serve 5e8 resident)")
+    ap.add_argument("--seed", type=int, default=2, help="full_sky and serve configs: seed")
     ap.add_argument("--input-order", choices=["random", "dec"], default="random",
                     help="full_sky config: record order of both catalogs (dec = sorted by "
-                         "declination; a diagnostic form, not the headline)")
+                         "declination; a diagnostic form)")
+    ap.add_argument("--no-full-sky", action="store_true",
+                    help="plane config: skip the 50M x 50M FP64 figures reported beside it")
+    ap.add_argument("--chunk-rows", type=int, default=0,
+                    help="pair kernel: A rows per chunk (a multiple of 64 up to 1024; 0: the "
+                         "library default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pair kernel: shapes picked by the library's tuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-dec-band", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU baseline: target length of the timed oracle run")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (for scaling PMC byte counters)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the library drives one GPU and has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
     if args.calibrate_copy:
         import torch
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -460,137 +758,8 @@ def main():
         print(json.dumps({"calibration": "device copy", "bytes_read": 1 << 30,
                           "bytes_written": 1 << 30, "launches": 3}), flush=True)
         return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the library drives one GPU and has no multi-GPU path "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    if args.config == "plane":
-        run_plane(args)
-        return
-    if args.config == "batch":
-        run_batch(args)
-        return
-    if args.config == "serve":
-        run_serve(args)
-        return
-
-    t0 = time.time()
-    A, B, r = synth.config_full_sky(seed=args.seed, n_a=args.n, n_b=args.n)
-    log("generated %d x %d sources in %.1f s" % (len(A), len(B), time.time() - t0))
-    if args.input_order == "dec":
-        # spatially ordered input, as survey catalogs are often stored (a diagnostic
-        # form; the headline workload keeps the random order)
-        A = A[np.argsort(A["dec"], kind="stable")]
-        B = B[np.argsort(B["dec"], kind="stable")]
-
-    ctx = xm.Context(0)
-    dA = ctx.to_device(A)
-    dB = ctx.to_device(B)
-
-    def step():
-        ca = ctx.catalog(dA, r)
-        cb = ctx.catalog(dB, r)
-        res = ctx.match(ca, cb, r)
-        n = res.count
-        stats = res.stats.as_dict()
-        res.close(); ca.close(); cb.close()
-        return n, stats
-
-    for i in range(args.warmup):
-        step()
-        log("warmup step %d done" % i)
-    # Per-kernel breakdown: events around every launch, on steps of their own.  Each
-    # timed launch adds two event records to the stream (~8 us per launch on
-    # MI355X), so the timed steps below carry events around the dominant kernel
-    # only.
-    ctx.synchronize()
-    ctx.reset_timing()
-    ctx.set_timing(True)
-    for i in range(BREAKDOWN_STEPS):
-        step()
-    ctx.set_timing(False)
-    ktimes = {}
-    for k in KERNELS:
-        tot, cnt = ctx.kernel_time(k)
-        if cnt:
-            ktimes[k] = {"total_ms": tot, "launches": cnt, "avg_ms": tot / cnt}
-    dom = max(ktimes.items(), key=lambda kv: kv[1]["total_ms"])[0]
-    log("breakdown over %d steps; dominant kernel %s" % (BREAKDOWN_STEPS, dom))
-
-    ctx.reset_timing()
-    ctx.set_timing(True, only=dom)
-    ctx.synchronize()
-    t_start = time.perf_counter()
-    join_ms = 0.0
-    n_match = 0
-    for i in range(args.steps):
-        n_match, st = step()
-        join_ms += st["join_ms"]
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ctx.set_timing(False)
-    log("timed %d steps in %.3f s" % (args.steps, elapsed))
-    dom_total_ms, dom_launches = ctx.kernel_time(dom)
-    dom_avg_ms = dom_total_ms / dom_launches
-
-    ca = ctx.catalog(dA, r)
-    cb = ctx.catalog(dB, r)
-    w = ctx.work_counts(ca, cb, r)
-    ca.close(); cb.close()
-    log("work counts: %s" % w)
-
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = w["w_cand"] / (ms_per_step / 1e3)
-    join_ms_avg = join_ms / args.steps
-
-    # the dominant kernel (by total device time in the breakdown), its average
-    # launch duration timed over the timed steps
-    if dom == "pair_fp64":
-        # 9 FP64 VALU operations per executed pair test (3 sub, 3 mul, 2 add, 1 cmp)
-        flops = w["w_tile"] * 9.0
-        ach = flops / (dom_avg_ms / 1e3) / 1e12
-        roof = {"kernel": dom, "bound": "valu", "achieved": ach, "peak": FP64_VECTOR_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_VECTOR_PEAK_TFLOPS, "traffic": None}
-    else:
-        per_launch = BYTES_PER_SOURCE.get(dom, 0.0) * args.n
-        ach = per_launch / (dom_avg_ms / 1e3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                "algorithmic_bytes_per_launch": per_launch}
-    roof["avg_launch_ms"] = dom_avg_ms
-    roof["launches_timed"] = dom_launches
-
-    sort_ms = sum(ktimes.get(k, {}).get("total_ms", 0.0)
-                  for k in ("radix_hist", "radix_offsets", "radix_scatter")) / BREAKDOWN_STEPS
-    passes = ktimes.get("radix_scatter", {}).get("launches", 0) / BREAKDOWN_STEPS / 2
-    # per catalog: histogram read 8, then per pass scatter read 8 + write 8; both catalogs
-    sort_bytes = 2 * args.n * (8 + passes * 16)
-
-    line = {
-        "metric": "pair-tests/s", "value": value, "unit": "pair-tests/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "full-sky plain FP64 zone join, 50M x 50M, r = h = 2 arcsec",
-                   "n_a": args.n, "n_b": args.n, "radius_arcsec": 2.0, "seed": args.seed,
-                   "input_order": args.input_order},
-        "matches": n_match, "matches_per_s": n_match / (ms_per_step / 1e3),
-        "join_ms": join_ms_avg, "join_pair_tests_per_s": w["w_cand"] / (join_ms_avg / 1e3),
-        "w_ref": w["w_ref"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
-        "sort_ms_per_step": sort_ms, "sort_passes_per_catalog": passes,
-        "sort_gb_per_s": sort_bytes / (sort_ms / 1e3) / 1e9 if sort_ms else None,
-        "kernels": ktimes, "kernels_from": "%d untimed steps with events around every launch"
-                                             % BREAKDOWN_STEPS,
-        "roofline": roof,
-    }
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline on %d threads..." % threads)
-        line["cpu_baseline"] = cpu_baseline(A, B, r, args.cpu_dec_band, threads)
-    else:
-        line["cpu_baseline"] = None
-    ctx.close()
-    print(json.dumps(line), flush=True)
+    {"plane": run_headline, "full_sky": run_full_sky, "batch": run_batch,
+     "serve": run_serve}[args.config](args)
 
 
 if __name__ == "__main__":

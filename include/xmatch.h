@@ -111,6 +111,11 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  *   "sort_wide_words" 1 = the catalog sort's tile look-back uses 64-bit words at
  *                every size (by default only from 2^30 rows up; 32-bit words
  *                below).  Default 0.
+ *   "chunk_rows", "autotune": the pair kernel's shape, see xm_ctx_tune_table.
+ *   "fp32_rows"  1 = catalog creation also writes each row's FP32 unit vector (16
+ *                bytes a row, in the same pass as the FP64 rows), as the FP32 filter
+ *                (XM_FP32_FILTER_FP64_RECHECK) reads them.  0 = they are made on a
+ *                catalog's first FP32 join, by a pass of their own.  Default 0.
  *   "sort_fail_test" 1 = every catalog sort of more than one tile (8192 rows)
  *                fails through its look-back's bounded wait, and catalog creation
  *                returns XM_EINTERNAL: for tests of that error path.  Default 0.
@@ -118,6 +123,25 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  * does. */
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
+/* The pair kernel's shape per job: "chunk_rows" (A rows per wavefront, a multiple
+ * of 64 up to 1024; 0 = 512) and "dense_min" above are the shape every join uses,
+ * unless "autotune" is 1.  Then a join first looks up each job's tuning key --
+ * log2 rows per occupied zone of its A and B segments (rounded) and nz =
+ * ceil(r / h) -- in the context's tune table.  For a key the table lacks, it times
+ * count-only pair launches of up to 16 of the batch's jobs with that key (the
+ * largest) for chunk rows {64, 128, 256, 512, 1024} x dense_min {4, 16, 64} and
+ * keeps the fastest shape.  The reference's kernel shape was fixed at compile time
+ * (Worker.cu:22,351-353).  Results never depend on the shape. */
+typedef struct {
+    uint32_t key;          /* log2 rows/zone of A | log2 rows/zone of B << 8 | nz << 16 */
+    uint32_t chunk_rows;   /* chosen A rows per chunk */
+    uint32_t dense_min;    /* chosen dense rule */
+    uint32_t sample_jobs;  /* jobs the shapes were timed on */
+    double tuned_ms;       /* pair kernel time of the chosen shape on the sample */
+    double default_ms;     /* the same for the default shape (the context's knobs) */
+} xm_tune_entry;
+/* The tune table: up to cap entries into out (may be NULL), their count in *n. */
+xm_status xm_ctx_tune_table(xm_ctx *ctx, xm_tune_entry *out, uint64_t cap, uint64_t *n);
 /* Free and total device memory; blocks the context caches for reuse count as free. */
 xm_status xm_ctx_mem_info(xm_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 /* Wait for all device work of the context. */

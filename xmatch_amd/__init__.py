@@ -44,7 +44,7 @@ EXPORTED = [
     "xm_result_jobs", "xm_result_job_device_ptr", "xm_work_counts_job",
     "xm_server_create", "xm_server_submit", "xm_server_poll", "xm_server_fetch",
     "xm_server_error", "xm_server_get_stats", "xm_server_destroy",
-    "xm_synth_fresh", "xm_synth_copies",
+    "xm_synth_fresh", "xm_synth_copies", "xm_ctx_tune_table",
 ]
 
 
@@ -98,6 +98,20 @@ def _job_array(jobs):
         arr[i].seg_b = int(b)
         arr[i].radius_deg = float(r)
     return arr
+
+
+class TuneEntry(ctypes.Structure):
+    """xm_tune_entry: one measured key of the context's tune table."""
+    _fields_ = [("key", ctypes.c_uint32), ("chunk_rows", ctypes.c_uint32),
+                ("dense_min", ctypes.c_uint32), ("sample_jobs", ctypes.c_uint32),
+                ("tuned_ms", ctypes.c_double), ("default_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["log2_rows_per_zone_a"] = self.key & 0xff
+        d["log2_rows_per_zone_b"] = (self.key >> 8) & 0xff
+        d["nz"] = self.key >> 16
+        return d
 
 
 class ServerConfig(ctypes.Structure):
@@ -199,6 +213,8 @@ def lib():
             "xm_server_error": ([vp, u64], ctypes.c_char_p),
             "xm_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
             "xm_server_destroy": ([vp], None),
+            "xm_ctx_tune_table": ([vp, ctypes.POINTER(TuneEntry), u64, ctypes.POINTER(u64)],
+                                  ctypes.c_int),
             "xm_synth_fresh": ([vp, ctypes.POINTER(Shape), u64, u64, u64, ctypes.c_int64, vp],
                                ctypes.c_int),
             "xm_synth_copies": ([vp, vp, u64, dbl, dbl, ctypes.POINTER(Shape), u64, u64, u64,
@@ -285,6 +301,14 @@ class Context:
         v = ctypes.c_int64()
         self._check(lib().xm_ctx_get_param(self._h, name.encode(), ctypes.byref(v)))
         return v.value
+
+    def tune_table(self):
+        """The measured pair-kernel shapes (param "autotune"), one dict per key."""
+        n = ctypes.c_uint64()
+        self._check(lib().xm_ctx_tune_table(self._h, None, 0, ctypes.byref(n)))
+        arr = (TuneEntry * max(n.value, 1))()
+        self._check(lib().xm_ctx_tune_table(self._h, arr, n.value, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def to_device(self, arr):
         """Copy a host array to a device buffer owned by the context pool."""
@@ -481,13 +505,16 @@ class Catalog:
         return {"first_row": f.value, "n": n.value, "zone_height_deg": h.value,
                 "zmin": z0.value, "zmax": z1.value}
 
-    def export(self):
+    def export(self, fields=("id", "x", "y", "z", "zone")):
+        """Sorted-order host arrays of the catalog: ids, FP64 unit vectors, zones
+        (only the fields named)."""
         n = self.info()["n"]
-        out = {"id": np.empty(n, np.int64), "x": np.empty(n), "y": np.empty(n),
-               "z": np.empty(n), "zone": np.empty(n, np.int32)}
-        self.ctx._check(lib().xm_catalog_export(self.ctx.handle, self._h, _ptr(out["id"]),
-                                                _ptr(out["x"]), _ptr(out["y"]), _ptr(out["z"]),
-                                                _ptr(out["zone"])))
+        types = {"id": np.int64, "x": np.float64, "y": np.float64, "z": np.float64,
+                 "zone": np.int32}
+        out = {f: np.empty(n, types[f]) for f in fields}
+        self.ctx._check(lib().xm_catalog_export(self.ctx.handle, self._h, _ptr(out.get("id")),
+                                                _ptr(out.get("x")), _ptr(out.get("y")),
+                                                _ptr(out.get("z")), _ptr(out.get("zone"))))
         return out
 
     def close(self):

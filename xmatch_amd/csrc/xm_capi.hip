@@ -64,20 +64,42 @@ int bit_length(uint32_t v) {
     return b;
 }
 
-// Build the A-side chunk table: every segment zone split into runs of at most
-// CHUNK_ROWS rows.  No synchronisation: the table is sized for the upper bound
-// n / CHUNK_ROWS + zones and the kernels read the true counts from the device.
-void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
-    if (A->chunks || A->n == 0) return;
+// Build the A-side chunk table: every zone of segment s split into runs of at most
+// rows[s] rows (a multiple of 64, at most MAX_CHUNK_ROWS).  A table built with
+// other rows is replaced.  No synchronisation: the table is sized for the upper
+// bound sum_s n_s / rows[s] + zones, and the kernels read the true counts from the
+// device.  The segment table's device copy carries the rows per segment.
+void ensure_chunks(xm_ctx* ctx, xm_catalog* A, const std::vector<uint32_t>& rows) {
+    if (A->n == 0) return;
+    bool same = A->chunks != nullptr;
+    for (size_t s = 0; s < A->seg.size(); s++) same = same && A->seg[s].chunk_rows == rows[s];
+    if (same) return;
+    dfree(ctx, A->chunks);
+    dfree(ctx, A->chunk_start);
+    A->chunks = nullptr;
+    A->chunk_start = nullptr;
+    uint64_t bound = 0;
+    for (size_t s = 0; s < A->seg.size(); s++) {
+        SegDesc& d = A->seg[s];
+        if (rows[s] == 0 || rows[s] % WAVE || rows[s] > uint32_t(MAX_CHUNK_ROWS))
+            throw std::logic_error("chunk rows must be a multiple of 64 up to 1024");
+        d.chunk_rows = rows[s];
+        if (d.n) bound += uint64_t(d.n) / d.chunk_rows + uint64_t(d.zmax - d.zmin + 1);
+    }
+    // the device copy of the segment table (the chunk kernels read chunk_rows), in
+    // stream order; the host table changes only in a later call, after this call
+    // has synchronised the stream
+    XM_HIP(hipMemcpyAsync(A->seg_dev, A->seg.data(), A->seg.size() * sizeof(SegDesc),
+                          hipMemcpyHostToDevice, ctx->stream));
     const uint64_t entries = A->n_zoff - 1;  // zone entries; the last zoff entry is n
+    const uint32_t S = uint32_t(A->seg.size());
     ScopedBlocks tmp(ctx);
     uint32_t* counts = tmp.alloc<uint32_t>(A->n_zoff * sizeof(uint32_t));
     launch_fill_u32(ctx, counts + entries, 1, 0u);
-    launch_chunk_counts(ctx, A->zoff, entries, counts);
+    launch_chunk_counts(ctx, A->zoff, entries, A->seg_dev, S, counts);
     scan_exclusive_u32(ctx, counts, A->n_zoff);
-    const uint64_t bound = A->n / CHUNK_ROWS + entries;
     Chunk* chunks = tmp.alloc<Chunk>(std::max<uint64_t>(bound, 1) * sizeof(Chunk));
-    launch_chunk_fill(ctx, A->zoff, counts, A->key, entries, chunks);
+    launch_chunk_fill(ctx, A->zoff, counts, A->key, entries, A->seg_dev, S, chunks);
     tmp.keep(chunks);
     tmp.keep(counts);
     A->chunks = chunks;
@@ -85,9 +107,9 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A) {
     A->chunk_start = counts;
 }
 
-// Upper bound on the chunks of segment s (rows / CHUNK_ROWS + zones).
-uint64_t seg_max_chunks(const SegDesc& s) {
-    return s.n ? uint64_t(s.n) / CHUNK_ROWS + uint64_t(s.zmax - s.zmin + 1) : 0;
+// Upper bound on the chunks of segment s at rows per chunk (rows / chunk + zones).
+uint64_t seg_max_chunks(const SegDesc& s, uint32_t rows) {
+    return s.n ? uint64_t(s.n) / rows + uint64_t(s.zmax - s.zmin + 1) : 0;
 }
 
 PairParams make_params(const xm_ctx* ctx, const SegDesc& A, const SegDesc& B, double radius_deg) {
@@ -140,6 +162,31 @@ size_t size_class(size_t bytes) {
     return 2 * p;
 }
 }  // namespace
+
+// The pair kernel's shape when no tuned shape applies: the context's knobs.
+JobShape default_shape(const xm_ctx* ctx) {
+    return JobShape{ctx->chunk_rows ? ctx->chunk_rows : uint32_t(CHUNK_ROWS), ctx->dense_min};
+}
+
+uint32_t tune_key(const SegDesc& a, const SegDesc& b, double radius_deg) {
+    auto density = [](const SegDesc& d) {
+        const double zones = d.n ? double(d.zmax - d.zmin + 1) : 1.0;
+        const double rows = std::max(1.0, double(d.n) / zones);
+        return uint32_t(std::min(31L, std::lround(std::log2(rows))));
+    };
+    const double nz = std::ceil(radius_deg / a.h_deg);
+    const uint32_t z = uint32_t(std::max(1.0, std::min(255.0, nz)));
+    return density(a) | density(b) << 8 | z << 16;
+}
+
+JobShape job_shape(const xm_ctx* ctx, const SegDesc& a, const SegDesc& b, double radius_deg) {
+    if (ctx->autotune) {
+        const uint32_t k = tune_key(a, b, radius_deg);
+        for (const auto& e : ctx->tune)
+            if (e.first == k) return JobShape{e.second.chunk_rows, e.second.dense_min};
+    }
+    return default_shape(ctx);
+}
 
 void* dalloc(xm_ctx* ctx, size_t bytes) {
     const size_t cls = size_class(bytes ? bytes : 1);
@@ -369,8 +416,9 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     tmp.free(scratch);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
+    if (ctx->fp32_rows) cat->row32 = dalloc(ctx, n * sizeof(Row32));
     launch_gather(ctx, d.recs, words, n, pl, ra_drop, zmin, static_cast<Row64*>(cat->row),
-                  cat->zoff, cat->n_zoff, words_alt);
+                  static_cast<Row32*>(cat->row32), cat->zoff, cat->n_zoff, words_alt);
     tmp.free(words);
     tmp.keep(words_alt);
     cat->key = words_alt;
@@ -444,8 +492,9 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
     radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits, digit_totals);
     tmp.free(digit_totals);
     cat->row = dalloc(ctx, n * sizeof(Row64));
+    if (ctx->fp32_rows) cat->row32 = dalloc(ctx, n * sizeof(Row32));
     launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row),
-                      cat->zoff, cat->n_zoff, words_alt);
+                      static_cast<Row32*>(cat->row32), cat->zoff, cat->n_zoff, words_alt);
     tmp.free(words);
     tmp.keep(words_alt);
     cat->key = words_alt;
@@ -476,25 +525,35 @@ struct JobTable {
 };
 
 // The table is uploaded asynchronously; t.host (or the context's pinned job stage)
-// must stay untouched until the stream is synchronised.
+// must stay untouched until the stream is synchronised.  shape[j] is job j's pair
+// kernel shape; an A segment takes the chunk rows of the first job that uses it
+// (segments no job uses keep the rows of their current chunk table).
 void make_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jobs,
-               uint64_t n_jobs, JobTable& t) {
+               uint64_t n_jobs, const std::vector<JobShape>& shape, JobTable& t) {
     if (n_jobs == 0) throw InvalidArg{"no jobs"};
     if (n_jobs >= 0xffffffffull) throw InvalidArg{"too many jobs"};
     const bool pinned = n_jobs * sizeof(JobDesc) <= PIN_JOB_BYTES;
     if (!pinned) t.host.resize(n_jobs);
     JobDesc* h = pinned ? reinterpret_cast<JobDesc*>(static_cast<char*>(ctx->pinned) + PIN_JOB_OFFSET)
                         : t.host.data();
-    uint64_t bound = 0;
+    std::vector<uint32_t> rows(A->seg.size(), 0);
     for (uint64_t j = 0; j < n_jobs; j++) {
         const xm_job& jb = jobs[j];
         if (jb.seg_a >= A->seg.size() || jb.seg_b >= B->seg.size())
             throw InvalidArg{"job " + std::to_string(j) + " names a segment the catalog lacks"};
+        if (!rows[jb.seg_a]) rows[jb.seg_a] = shape[j].chunk_rows;
+    }
+    for (size_t s = 0; s < rows.size(); s++)
+        if (!rows[s]) rows[s] = A->seg[s].chunk_rows ? A->seg[s].chunk_rows : default_shape(ctx).chunk_rows;
+    uint64_t bound = 0;
+    for (uint64_t j = 0; j < n_jobs; j++) {
+        const xm_job& jb = jobs[j];
         const SegDesc& sa = A->seg[jb.seg_a];
         const SegDesc& sb = B->seg[jb.seg_b];
         check_radius(jb.radius_deg, sa.h_deg, sb.h_deg);
         JobDesc& d = h[j];
         d.pp = make_params(ctx, sa, sb, jb.radius_deg);
+        d.pp.dense_min = shape[j].dense_min;
         d.b_key_prefix = uint64_t(jb.seg_b) << KEY_SEG_SHIFT;
         d.a_zoff_lo = sa.zoff_base;
         d.a_zoff_hi = sa.zoff_base + (sa.n ? uint32_t(sa.zmax - sa.zmin + 1) : 0u);
@@ -502,12 +561,12 @@ void make_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
         d.b_zmin = sb.n ? sb.zmin : 0;
         d.b_zmax = sb.n ? sb.zmax : -1;
         d.a_chunk_base = 0;
-        if (sb.n) bound += seg_max_chunks(sa);
+        if (sb.n) bound += seg_max_chunks(sa, rows[jb.seg_a]);
     }
     if (bound >= 0xffffffffull) throw InvalidArg{"batch too large (more than 2^32 wavefronts)"};
     t.n_jobs = uint32_t(n_jobs);
     t.max_waves = bound;
-    ensure_chunks(ctx, A);
+    ensure_chunks(ctx, A, rows);
     t.jobs = static_cast<JobDesc*>(dalloc(ctx, n_jobs * sizeof(JobDesc)));
     t.prefix = static_cast<uint32_t*>(dalloc(ctx, (n_jobs + 1) * sizeof(uint32_t)));
     XM_HIP(hipMemcpyAsync(t.jobs, h, n_jobs * sizeof(JobDesc), hipMemcpyHostToDevice, ctx->stream));
@@ -619,6 +678,16 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "sort_fail_test") {
             if (value != 0 && value != 1) throw InvalidArg{"sort_fail_test must be 0 or 1"};
             ctx->sort_fail_test = value != 0;
+        } else if (n == "fp32_rows") {
+            if (value != 0 && value != 1) throw InvalidArg{"fp32_rows must be 0 or 1"};
+            ctx->fp32_rows = value != 0;
+        } else if (n == "autotune") {
+            if (value != 0 && value != 1) throw InvalidArg{"autotune must be 0 or 1"};
+            ctx->autotune = value != 0;
+        } else if (n == "chunk_rows") {
+            if (value < 0 || value > MAX_CHUNK_ROWS || value % WAVE)
+                throw InvalidArg{"chunk_rows must be 0 or a multiple of 64 up to 1024"};
+            ctx->chunk_rows = uint32_t(value);
         } else {
             throw InvalidArg{"unknown parameter '" + n + "'"};
         }
@@ -634,8 +703,23 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
         else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
         else if (n == "sort_wide_words") *value = ctx->sort_wide_words ? 1 : 0;
         else if (n == "sort_fail_test") *value = ctx->sort_fail_test ? 1 : 0;
+        else if (n == "fp32_rows") *value = ctx->fp32_rows ? 1 : 0;
+        else if (n == "chunk_rows") *value = ctx->chunk_rows ? ctx->chunk_rows : CHUNK_ROWS;
+        else if (n == "autotune") *value = ctx->autotune ? 1 : 0;
         else throw InvalidArg{"unknown parameter '" + n + "'"};
     });
+}
+
+xm_status xm_ctx_tune_table(xm_ctx* ctx, xm_tune_entry* out, uint64_t cap, uint64_t* n) {
+    if (!ctx || !n) return XM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *n = ctx->tune.size();
+    for (uint64_t i = 0; out && i < cap && i < ctx->tune.size(); i++) {
+        const auto& e = ctx->tune[i];
+        out[i] = xm_tune_entry{e.first, e.second.chunk_rows, e.second.dense_min,
+                               e.second.sample_jobs, e.second.tuned_ms, e.second.default_ms};
+    }
+    return XM_OK;
 }
 
 xm_status xm_ctx_release_memory(xm_ctx* ctx, uint64_t* cached_bytes) {
@@ -867,14 +951,118 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
 namespace {
 
 // Above this much staging (24 bytes a slot) a join stages row indices (8 bytes a
-// slot) and the compaction gathers the ids and distances from the rows.
-constexpr uint64_t WIDE_STAGING_BYTES = uint64_t(8) << 30;
+// slot) and the compaction gathers the ids and distances from the rows.  Narrow
+// slots cost two random 32-byte row reads per match in the compaction: on the
+// 200M x 200M plane join at 1" (1.0e8 matches, 4e8 default slots = 9.6 GB wide)
+// the compaction took 2.97 ms with narrow slots under an 8 GB limit.
+constexpr uint64_t WIDE_STAGING_BYTES = uint64_t(64) << 30;
 
 void free_slots(xm_ctx* ctx, RegionOut& ro) {
     dfree(ctx, ro.id_a); dfree(ctx, ro.id_b); dfree(ctx, ro.d2);
     dfree(ctx, ro.ra); dfree(ctx, ro.rb); dfree(ctx, ro.job);
     ro.id_a = nullptr; ro.id_b = nullptr; ro.d2 = nullptr;
     ro.ra = nullptr; ro.rb = nullptr; ro.job = nullptr;
+}
+
+// ---- autotuner: the pair kernel's shape per tuning key, by measurement.
+// Candidate shapes: chunk rows x dense rule.  Each is timed on a sample of the
+// batch's jobs of the key (up to TUNE_SAMPLE, the largest by nA * nB) as a
+// count-only pair launch: regions and spill pools of zero slots, so every hit is
+// counted and none is written.  The fastest of two launches per shape counts.
+constexpr uint32_t TUNE_ROWS[] = {64, 128, 256, 512, 1024};
+constexpr uint32_t TUNE_DENSE[] = {4, 16, 64};
+constexpr size_t TUNE_SAMPLE = 16;
+struct xm_tune_record {
+    uint32_t rows, dense_min;
+    double ms;
+};
+
+void tune_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jobs,
+               uint64_t n_jobs, bool f32) {
+    // the jobs of every key the table lacks
+    std::vector<std::pair<uint32_t, std::vector<uint64_t>>> todo;
+    for (uint64_t j = 0; j < n_jobs; j++) {
+        if (jobs[j].seg_a >= A->seg.size() || jobs[j].seg_b >= B->seg.size()) continue;
+        const SegDesc& sa = A->seg[jobs[j].seg_a];
+        const SegDesc& sb = B->seg[jobs[j].seg_b];
+        if (!sa.n || !sb.n || sa.h_deg != sb.h_deg || !(jobs[j].radius_deg > 0.0)) continue;
+        const uint32_t k = tune_key(sa, sb, jobs[j].radius_deg);
+        bool known = false;
+        for (const auto& e : ctx->tune) known = known || e.first == k;
+        if (known) continue;
+        bool found = false;
+        for (auto& t : todo)
+            if (t.first == k) { t.second.push_back(j); found = true; }
+        if (!found) todo.push_back({k, {j}});
+    }
+    if (todo.empty()) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XM_HIP(hipEventCreate(&e0));
+    XM_HIP(hipEventCreate(&e1));
+    ScopedBlocks tmp(ctx);
+    const uint32_t P = SPILL_POOLS;
+    uint64_t* counters = tmp.alloc<uint64_t>(size_t(MAX_REGIONS + P) * REGION_STRIDE * 8);
+    uint64_t* zeros = tmp.alloc<uint64_t>(size_t(MAX_REGIONS + P) * 8);
+    launch_fill_u64(ctx, zeros, MAX_REGIONS + P, 0);
+    try {
+        for (auto& t : todo) {
+            std::vector<uint64_t>& js = t.second;
+            auto work = [&](uint64_t j) {
+                return double(A->seg[jobs[j].seg_a].n) * double(B->seg[jobs[j].seg_b].n);
+            };
+            std::stable_sort(js.begin(), js.end(),
+                             [&](uint64_t x, uint64_t y) { return work(x) > work(y); });
+            if (js.size() > TUNE_SAMPLE) js.resize(TUNE_SAMPLE);
+            std::vector<xm_job> sample;
+            for (uint64_t j : js) sample.push_back(jobs[j]);
+            xm_tune_record best{};
+            bool have = false;
+            double default_ms = -1.0;
+            for (uint32_t rows : TUNE_ROWS) {
+                for (uint32_t dmin : TUNE_DENSE) {
+                    JobTable jt;
+                    make_jobs(ctx, A, B, sample.data(), sample.size(),
+                              std::vector<JobShape>(sample.size(), JobShape{rows, dmin}), jt);
+                    RegionOut ro{};
+                    ro.n_regions = std::max<uint32_t>(
+                        1, std::min<uint32_t>(MAX_REGIONS, pair_blocks(std::max<uint64_t>(jt.max_waves, 1))));
+                    ro.counters = counters;
+                    ro.base = zeros;
+                    ro.cap = zeros;
+                    ro.spill_cap = 0;
+                    ro.wide = false;
+                    float best_ms = 1e30f;
+                    for (int rep = 0; rep < 2; rep++) {
+                        launch_fill_u64(ctx, counters, size_t(ro.n_regions + P) * REGION_STRIDE, 0);
+                        XM_HIP(hipEventRecord(e0, ctx->stream));
+                        launch_pair(ctx, A, B, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
+                        XM_HIP(hipEventRecord(e1, ctx->stream));
+                        XM_HIP(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        XM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        best_ms = std::min(best_ms, ms);
+                    }
+                    dfree(ctx, jt.jobs);
+                    dfree(ctx, jt.prefix);
+                    const JobShape def = default_shape(ctx);
+                    if (rows == def.chunk_rows && dmin == def.dense_min) default_ms = best_ms;
+                    if (!have || best_ms < best.ms) {
+                        best = xm_tune_record{rows, dmin, double(best_ms)};
+                        have = true;
+                    }
+                }
+            }
+            xm_ctx::TuneEntry e{best.rows, best.dense_min, uint32_t(sample.size()), best.ms,
+                                default_ms};
+            ctx->tune.push_back({t.first, e});
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    XM_HIP(hipEventDestroy(e0));
+    XM_HIP(hipEventDestroy(e1));
 }
 
 // The join of a batch of jobs, with overflow-and-retry.  batch = keep the job
@@ -902,7 +1090,6 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
         const bool f32 = precision == XM_FP32_FILTER_FP64_RECHECK;
         xm_catalog* A = const_cast<xm_catalog*>(a);
         xm_catalog* Bm = const_cast<xm_catalog*>(b);
-        make_jobs(ctx, A, b, jobs, n_jobs, jt);
         if (f32) {
             for (xm_catalog* c : {A, Bm}) {
                 if (c->row32 || c->n == 0) continue;
@@ -911,6 +1098,15 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
                                 static_cast<Row32*>(c->row32));
             }
         }
+        if (ctx->autotune) tune_jobs(ctx, A, b, jobs, n_jobs, f32);
+        std::vector<JobShape> shape(n_jobs);
+        for (uint64_t j = 0; j < n_jobs; j++) {
+            if (jobs[j].seg_a >= A->seg.size() || jobs[j].seg_b >= b->seg.size())
+                throw InvalidArg{"job " + std::to_string(j) + " names a segment the catalog lacks"};
+            shape[j] = job_shape(ctx, A->seg[jobs[j].seg_a], b->seg[jobs[j].seg_b],
+                                 jobs[j].radius_deg);
+        }
+        make_jobs(ctx, A, b, jobs, n_jobs, shape, jt);
         uint64_t cap = capacity_hint;
         if (!cap) {
             // the reference's default, 2 * max(nA, nB) per job (Worker.cu:295-300)

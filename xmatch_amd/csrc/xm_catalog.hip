@@ -345,6 +345,20 @@ __device__ inline void store_row_nt(Row64* p, const Row64& r) {
     __builtin_nontemporal_store(b, reinterpret_cast<d2*>(p) + 1);
 }
 
+// The FP32 row of a unit vector (each component rounded to nearest), for catalogs
+// built for the FP32 filter: written by the gather next to the FP64 row, so the
+// FP32 form costs 16 more bytes of writes per row rather than a pass of its own
+// (a separate pass reads the 32-byte rows back and writes the 16-byte ones).
+__device__ inline void store_row32_nt(Row32* p, const Row64& r) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 v;
+    v.x = float(r.x);
+    v.y = float(r.y);
+    v.z = float(r.z);
+    v.w = 0.0f;
+    __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
+}
+
 // k_gather takes one row per thread in blocks of GATHER_TPB: 50M random-order
 // records, 256 threads 1.61 ms, 512 1.60 ms, 1024 1.59 ms.  k_gather_seg keeps
 // blocks of TPB: the 10,000-segment batch's gather took 49.5 ms with 256 threads and
@@ -354,6 +368,7 @@ constexpr int GATHER_TPB = 1024;
 __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
                                                     const uint64_t* __restrict__ words, uint32_t n,
                                                     Row64* __restrict__ rows,
+                                                    Row32* __restrict__ rows32,
                                                     uint32_t* __restrict__ zoff, uint32_t n_zoff,
                                                     uint64_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ sort_err) {
@@ -382,6 +397,7 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
     unit_vector(rec.ra_deg, rec.dec_deg, &r.x, &r.y, &r.z);
     r.id = rec.id;
     store_row_nt(&rows[i], r);
+    if (rows32) store_row32_nt(&rows32[i], r);
     const uint64_t zone = uint64_t(seg[s].zone_base) + uint64_t(zrel);
     __builtin_nontemporal_store(
         (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb)), &keys[i]);
@@ -394,6 +410,7 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
                                                 const uint64_t* __restrict__ words, uint32_t n,
                                                 PackLayout pl, int ra_drop, int32_t zmin,
                                                 Row64* __restrict__ rows,
+                                                Row32* __restrict__ rows32,
                                                 uint32_t* __restrict__ zoff, uint32_t n_zoff,
                                                 uint64_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ sort_err) {
@@ -414,6 +431,7 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
     unit_vector(ra, dec, &r.x, &r.y, &r.z);
     r.id = oid;
     store_row_nt(&rows[i], r);
+    if (rows32) store_row32_nt(&rows32[i], r);
     __builtin_nontemporal_store(
         (uint64_t(zone) << RA_BITS) | ((ra_q >> ra_drop) << (RA_BITS - pl.ra_bits + ra_drop)),
         &keys[i]);
@@ -423,30 +441,52 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
 // for z in [zmin(s), zmax(s)]; the entry after a segment's last is the next
 // segment's first row, and zoff[n_zoff - 1] = n.  Row i writes the entries of the
 // zones in (zone(i-1), zone(i)] of its segment (from zmin(s) at a segment start).
+//
+// Chunks: zoff entry k (a zone of segment s) holds ceil(rows / chunk_rows(s))
+// chunks.  The entry's segment is the last one whose zoff_base <= k: an empty
+// segment shares its base with the next, so the search lands on the segment that
+// owns the entry.
+__device__ inline uint32_t chunk_rows_of_entry(const SegDesc* __restrict__ seg, uint32_t n_seg,
+                                               uint32_t k) {
+    uint32_t lo = 0, hi = n_seg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (seg[mid].zoff_base <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return seg[lo].chunk_rows;
+}
+
 __global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict__ zoff,
                                                       uint32_t n_entries,
+                                                      const SegDesc* __restrict__ seg,
+                                                      uint32_t n_seg,
                                                       uint32_t* __restrict__ counts) {
     const uint32_t k = blockIdx.x * TPB + threadIdx.x;
     if (k >= n_entries) return;
     const uint32_t rows = zoff[k + 1] - zoff[k];
-    counts[k] = (rows + CHUNK_ROWS - 1) / CHUNK_ROWS;
+    const uint32_t cr = chunk_rows_of_entry(seg, n_seg, k);
+    counts[k] = (rows + cr - 1) / cr;
 }
 
 __global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__ zoff,
                                                     const uint32_t* __restrict__ cstart,
                                                     const uint64_t* __restrict__ keys,
                                                     uint32_t n_entries,
+                                                    const SegDesc* __restrict__ seg,
+                                                    uint32_t n_seg,
                                                     Chunk* __restrict__ chunks) {
     const uint32_t k = blockIdx.x * TPB + threadIdx.x;
     if (k >= n_entries) return;
     const uint32_t r0 = zoff[k], r1 = zoff[k + 1];
     if (r1 <= r0) return;
+    const uint32_t cr = chunk_rows_of_entry(seg, n_seg, k);
     const int32_t zone = key_zone(keys[r0]);
     uint32_t c = cstart[k];
-    for (uint32_t r = r0; r < r1; r += CHUNK_ROWS, c++) {
+    for (uint32_t r = r0; r < r1; r += cr, c++) {
         Chunk ch;
         ch.start = r;
-        ch.count = (r1 - r) < uint32_t(CHUNK_ROWS) ? (r1 - r) : uint32_t(CHUNK_ROWS);
+        ch.count = (r1 - r) < cr ? (r1 - r) : cr;
         ch.zone = zone;
         ch.pad = 0;
         chunks[c] = ch;
@@ -512,15 +552,15 @@ void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, in
 }
 
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
-                   PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, uint32_t* zoff,
-                   uint64_t n_zoff, uint64_t* keys) {
+                   PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
+                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys) {
     if (n == 0) return;
     const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather, dim3(uint32_t((n + GATHER_TPB - 1) / GATHER_TPB)),
                            dim3(GATHER_TPB), 0, ctx->stream, recs, words,
-                           uint32_t(n), pl, ra_drop, zmin, rows, zoff, uint32_t(n_zoff), keys,
-                           err);
+                           uint32_t(n), pl, ra_drop, zmin, rows, rows32, zoff, uint32_t(n_zoff),
+                           keys, err);
     });
 }
 
@@ -545,29 +585,32 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 }
 
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
-                       uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys) {
+                       uint64_t n, Row64* rows, Row32* rows32, uint32_t* zoff, uint64_t n_zoff,
+                       uint64_t* keys) {
     if (n == 0) return;
     const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather_seg, grid_for(n), dim3(TPB), 0, ctx->stream, seg, seg_shift,
-                           words, uint32_t(n), rows, zoff, uint32_t(n_zoff), keys, err);
+                           words, uint32_t(n), rows, rows32, zoff, uint32_t(n_zoff), keys, err);
     });
 }
 
-void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries, uint32_t* counts) {
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries,
+                         const SegDesc* seg, uint32_t n_seg, uint32_t* counts) {
     if (n_entries == 0) return;
     launch_timed(ctx, "chunk_counts", [&] {
         hipLaunchKernelGGL(k_chunk_counts, grid_for(n_entries), dim3(TPB), 0, ctx->stream, zoff,
-                           uint32_t(n_entries), counts);
+                           uint32_t(n_entries), seg, n_seg, counts);
     });
 }
 
 void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
-                       const uint64_t* keys, uint64_t n_entries, Chunk* chunks) {
+                       const uint64_t* keys, uint64_t n_entries, const SegDesc* seg,
+                       uint32_t n_seg, Chunk* chunks) {
     if (n_entries == 0) return;
     launch_timed(ctx, "chunk_fill", [&] {
         hipLaunchKernelGGL(k_chunk_fill, grid_for(n_entries), dim3(TPB), 0, ctx->stream, zoff,
-                           cstart, keys, uint32_t(n_entries), chunks);
+                           cstart, keys, uint32_t(n_entries), seg, n_seg, chunks);
     });
 }
 

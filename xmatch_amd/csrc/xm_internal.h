@@ -64,6 +64,15 @@ struct xm_ctx {
     int key_ra_bits = 24;  // upper bound on the ra bits of a catalog's sort key
     bool sort_wide_words = false;  // 64-bit look-back words at every sort size
     bool sort_fail_test = false;   // sorts fail by their look-back limit (error-path tests)
+    bool fp32_rows = false;        // catalog builds also write the FP32 rows (gather pass)
+    uint32_t chunk_rows = 0;       // A rows per pair-kernel chunk (0: CHUNK_ROWS)
+    bool autotune = false;         // joins pick their shapes by measurement (tune table)
+    // tune table: key (tune_key) -> measured shape, its time and the default's
+    struct TuneEntry {
+        uint32_t chunk_rows, dense_min, sample_jobs;
+        double tuned_ms, default_ms;
+    };
+    std::vector<std::pair<uint32_t, TuneEntry>> tune;
 };
 
 namespace xm {
@@ -84,6 +93,8 @@ struct SegDesc {
     // build word layout: zone - zone_base in zone_bits above ra_bits above idx_bits
     int32_t zone_base;
     int32_t zone_bits, ra_bits, idx_bits;
+    uint32_t chunk_rows;  // A side: rows per chunk of the current chunk table (0: none yet)
+    uint32_t pad_;
 };
 
 }  // namespace xm
@@ -95,7 +106,9 @@ struct xm_catalog {
     xm::SegDesc* seg_dev = nullptr;  // device copy
     uint64_t* key = nullptr;         // sorted catalog keys (seg << 48 | zone << 24 | ra)
     void* row = nullptr;             // xm::Row64[n]: FP64 unit vector + id, sorted order
-    void* row32 = nullptr;           // xm::Row32[n]: FP32 unit vectors, made on first FP32 join
+    void* row32 = nullptr;           // xm::Row32[n]: FP32 unit vectors, written by the gather
+                                     // when the context's fp32_rows is set, else made on the
+                                     // first FP32 join
     uint32_t* zoff = nullptr;        // [n_zoff]: first row of each segment zone, then n
     uint64_t n_zoff = 0;             // sum over segments of (zmax - zmin + 1), plus 1
     // A-side row chunks (built on first use as the A catalog of a join): the
@@ -119,7 +132,14 @@ struct xm_result {
 namespace xm {
 
 constexpr int WAVE = 64;
-constexpr int CHUNK_ROWS = 256;  // A rows per wavefront in the pair kernel
+// A rows per wavefront in the pair kernel: a chunk is a run of at most chunk_rows
+// consecutive rows of one zone, chunk_rows a multiple of 64 up to MAX_CHUNK_ROWS,
+// chosen per A segment (default CHUNK_ROWS).  A zone of k chunks scans its
+// neighbour B zones k times: the 200M x 200M plane join at 1" (309 rows per zone on
+// average, up to ~505) took 7.09 ms in the pair kernel with 256-row chunks and
+// 5.93 ms with 512 (FP64, same box).
+constexpr int CHUNK_ROWS = 512;
+constexpr int MAX_CHUNK_ROWS = 1024;
 
 // One catalog row on the device: FP64 unit vector and id in one 32-byte line, so
 // a candidate B row is a single aligned fetch.
@@ -134,7 +154,7 @@ struct Row32 {
     uint32_t pad;
 };
 
-// A run of at most CHUNK_ROWS consecutive A rows of one zone of one segment.
+// A run of at most chunk_rows consecutive A rows of one zone of one segment.
 struct Chunk {
     uint32_t start;
     uint32_t count;
@@ -289,21 +309,26 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
                       uint64_t* words, int lo_bit = 0, int n_pass = 0,
                       uint32_t* digit_totals = nullptr);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
-// carries), keys[i] (the catalog key) and every segment's zone offsets
+// carries), rows32[i] when rows32 is not null (the FP32 unit vector, for the FP32
+// filter), keys[i] (the catalog key) and every segment's zone offsets
 // zoff[zoff_base + z - zmin] (first row of zone z; the entry after a segment's
 // last zone is the next segment's first row, zoff[n_zoff - 1] = n).  words and
 // keys are distinct buffers (a row reads its left neighbour's word).  The
 // one-segment form takes the layout by value; the many-segment form reads it
 // from the segment table.
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
-                   PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, uint32_t* zoff,
-                   uint64_t n_zoff, uint64_t* keys);
+                   PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
+                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys);
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
-                       uint64_t n, Row64* rows, uint32_t* zoff, uint64_t n_zoff, uint64_t* keys);
-// counts[k] = chunks of zoff entry k (k < n_entries); chunks from the scanned counts.
-void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries, uint32_t* counts);
+                       uint64_t n, Row64* rows, Row32* rows32, uint32_t* zoff, uint64_t n_zoff,
+                       uint64_t* keys);
+// counts[k] = chunks of zoff entry k (k < n_entries) at its segment's chunk_rows;
+// chunks from the scanned counts.
+void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries,
+                         const SegDesc* seg, uint32_t n_seg, uint32_t* counts);
 void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
-                       const uint64_t* keys, uint64_t n_entries, Chunk* chunks);
+                       const uint64_t* keys, uint64_t n_entries, const SegDesc* seg,
+                       uint32_t n_seg, Chunk* chunks);
 void launch_fill_u32(xm_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 void launch_fill_u64(xm_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v);
 void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
@@ -362,6 +387,21 @@ struct JobDesc {
     int32_t b_zmin, b_zmax;
     uint32_t a_chunk_base;     // first chunk of the A segment (device-filled)
 };
+// The pair kernel's shape for one job: A rows per chunk (a multiple of 64 up to
+// MAX_CHUNK_ROWS) and the dense rule (PairParams::dense_min).
+struct JobShape {
+    uint32_t chunk_rows;
+    uint32_t dense_min;
+};
+JobShape default_shape(const xm_ctx* ctx);
+// The shape a join of segment a with segment b at radius_deg runs with: the tuned
+// shape of its key when the context tunes and has measured that key, else the
+// default shape.
+JobShape job_shape(const xm_ctx* ctx, const SegDesc& a, const SegDesc& b, double radius_deg);
+// Tuning key of a job: log2 rows per occupied zone of each segment (rounded,
+// 0..31) and nz = ceil(r / h) (1..255): da | db << 8 | nz << 16.
+uint32_t tune_key(const SegDesc& a, const SegDesc& b, double radius_deg);
+
 // Per job: a_chunk_base = cstart[a_zoff_lo], waves[j] = its chunk count.
 void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t* cstart,
                      uint32_t* waves);

@@ -2,7 +2,8 @@
 //
 // Replaces the per-zone-pair launch loop of Worker::Match and xmatch_kernel
 // (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
-// into chunks of up to 256 consecutive rows of one zone; each wavefront owns one
+// into chunks of up to chunk_rows (64..1024, per A segment; 256 by default)
+// consecutive rows of one zone; each wavefront owns one
 // chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
 // alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  Each
 // chunk is joined in one of two modes, chosen per chunk (chunk_is_dense):
@@ -12,9 +13,9 @@
 //          within +-alpha of its ra and tests only those.  At zone height = radius
 //          a zone holds a few hundred sources over 360 deg, so a chunk spans most
 //          of the circle while alpha is arcseconds: a dense test of every (A, B)
-//          pair of the window would almost always be wasted.  With 256-row chunks
-//          a zone is usually one chunk, so each B row of its neighbour zones is
-//          examined about once per A zone.
+//          pair of the window would almost always be wasted.  When a zone is one
+//          chunk, each B row of its neighbour zones is examined once per A zone;
+//          a zone of k chunks examines them k times.
 //  dense   (pole rule, windows covering the circle, clustered data) 64 rows at a
 //          time, one A row per lane in registers; the window's B rows are walked
 //          in wave-uniform order as wide scalar (SGPR) loads, every lane testing
@@ -428,7 +429,7 @@ __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, ui
     if (x.s_aq[c] < lo_q) c++;
     for (uint32_t k = 0;; k++) {
         const uint32_t cc = c + k;
-        const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(CHUNK_ROWS) ? cc : 0] <= hi_q;
+        const bool act = in && cc < x.cnt && x.s_aq[cc < uint32_t(MAX_CHUNK_ROWS) ? cc : 0] <= hi_q;
         const uint64_t m = __ballot(act);
         if (!m) break;
         if (act) {
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
-    __shared__ int32_t s_aq[PK_WAVES][CHUNK_ROWS];
+    __shared__ int32_t s_aq[PK_WAVES][MAX_CHUNK_ROWS];
     __shared__ uint32_t s_qa[PK_WAVES][CQ];
     __shared__ uint32_t s_qb[PK_WAVES][CQ];
 
@@ -518,14 +519,18 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const uint32_t c = jobs[jid].a_chunk_base + (g - job_prefix[jid]);
 
     const uint32_t start = uni(chunks[c].start);
-    const uint32_t cnt = uni(chunks[c].count);
+    const uint32_t cnt = uni(chunks[c].count);  // 1 .. MAX_CHUNK_ROWS
     const int32_t za = int32_t(uni(uint32_t(chunks[c].zone)));
+    // the sparse search probes indices below 2 * half, half the largest power of
+    // two below cnt (1 for cnt <= 2): 2 * half <= MAX_CHUNK_ROWS
+    uint32_t half = 1;
+    while (2 * half < cnt) half *= 2;
 
-    // the chunk's ra quanta in LDS, padded with INT_MAX to CHUNK_ROWS
-#pragma unroll
-    for (uint32_t k = 0; k < uint32_t(CHUNK_ROWS) / WAVE; k++) {
-        const uint32_t r = k * WAVE + lane;
-        s_aq[w][r] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
+    // the chunk's ra quanta in LDS, padded with INT_MAX to 2 * half
+    for (uint32_t r0 = 0; r0 < 2 * half || r0 < cnt; r0 += WAVE) {
+        const uint32_t r = r0 + lane;
+        if (r < uint32_t(MAX_CHUNK_ROWS))
+            s_aq[w][r] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
     }
     wave_lds_sync();
     const uint64_t q_lo = uint32_t(s_aq[w][0]);
@@ -571,8 +576,6 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
         return;
     }
 
-    uint32_t half = 1;
-    while (2 * half < cnt) half *= 2;
     const SparseCtx x{start, cnt, half, int32_t(aq), s_aq[w]};
     CandQ q{s_qa[w], s_qb[w], 0, 0};
     for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {

@@ -7,9 +7,11 @@
 //            the pairs that reach the reference's chord test (Worker.cu:89-122),
 //            alpha = calc_alpha (Common.cu:16-23) without widening, ra = atan2(y, x)
 //            of the rows' unit vectors
-//   w_tile = sum over A chunks of rows(chunk) * B rows in the chunk's windows
-//            the tests the pair kernel executes (same integer window logic as
-//            the kernel: join_quanta, window_intervals, chunk_is_dense)
+//   w_tile = the tests the pair kernel executes under the job's shape (chunk rows
+//            and dense rule, job_shape): per dense chunk, rows * B rows in its
+//            windows; per sparse chunk, the chunk rows within each window B row's
+//            search range (same integer window logic as the kernel: join_quanta,
+//            window_intervals, chunk_is_dense)
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -112,6 +114,9 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
         const double atan_sin_theta = std::atan(std::sin(radius_deg / RAD2DEG));
         const int64_t gran = std::max(a_->ra_gran, b_->ra_gran);
         const int64_t full = int64_t(1) << RA_BITS;
+        // the shape the join of this job runs with
+        const JobShape shape = job_shape(ctx, A.d, B.d, radius_deg);
+        const uint64_t chunk_rows = shape.chunk_rows;
 
         std::atomic<int64_t> next{a_->n ? a_->zmin : 1};
         const int64_t zlast = a_->n ? a_->zmax : 0;
@@ -151,11 +156,11 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
                         }
                     }
                     // w_tile: the kernel's chunks, windows and per-chunk mode
-                    for (uint64_t s = a0; s < a1; s += CHUNK_ROWS) {
-                        const uint64_t cnt = std::min<uint64_t>(CHUNK_ROWS, a1 - s);
+                    for (uint64_t s = a0; s < a1; s += chunk_rows) {
+                        const uint64_t cnt = std::min<uint64_t>(chunk_rows, a1 - s);
                         const uint64_t q_lo = ka[s] & RA_MASK, q_hi = ka[s + cnt - 1] & RA_MASK;
                         const bool dense =
-                            chunk_is_dense(q_lo, q_hi, aq, ctx->dense_min, uint32_t(cnt));
+                            chunk_is_dense(q_lo, q_hi, aq, shape.dense_min, uint32_t(cnt));
                         uint64_t lo[2], hi[2];
                         int32_t shift[2];
                         if (dense) {
@@ -178,7 +183,7 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
                             const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
                             const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
                             // sparse: chunk rows within +-aq of each B row's quantum
-                            int32_t qs[CHUNK_ROWS];
+                            int32_t qs[MAX_CHUNK_ROWS];
                             for (uint64_t k = 0; k < cnt; k++) qs[k] = int32_t(ka[s + k] & RA_MASK);
                             for (uint64_t jj = j0; jj < j1; jj++) {
                                 const int32_t qb = int32_t(kb[jj] & RA_MASK) + shift[t2];
