@@ -19,41 +19,19 @@ using namespace xm;
 
 namespace {
 
-thread_local std::string g_tls_error = "no error";
 // pinned host scratch: small device->host reads at the front, then a stage for
 // the job table of small batches (one plain join included)
-constexpr size_t PIN_JOB_OFFSET = 4096;
 constexpr size_t PIN_JOB_BYTES = 64 * 1024;
 constexpr size_t PINNED_BYTES = PIN_JOB_OFFSET + PIN_JOB_BYTES;
 
-template <typename F>
-xm_status guarded(xm_ctx* ctx, F&& f) {
-    try {
-        if (ctx) XM_HIP(hipSetDevice(ctx->device));
-        f();
-        return XM_OK;
-    } catch (const InvalidArg& e) {
-        if (ctx) ctx->last_error = e.what;
-        g_tls_error = e.what;
-        return XM_EINVAL;
-    } catch (const OutOfMemory& e) {
-        if (ctx) ctx->last_error = e.what;
-        g_tls_error = e.what;
-        return XM_ENOMEM;
-    } catch (const DeviceError& e) {
-        const std::string m = e.what;
-        if (ctx) ctx->last_error = m;
-        g_tls_error = m;
-        return e.code == hipErrorOutOfMemory ? XM_ENOMEM : XM_EDEVICE;
-    } catch (const std::bad_alloc&) {
-        if (ctx) ctx->last_error = "host allocation failed";
-        g_tls_error = "host allocation failed";
-        return XM_ENOMEM;
-    } catch (const std::exception& e) {
-        if (ctx) ctx->last_error = e.what();
-        g_tls_error = e.what();
-        return XM_EINTERNAL;
-    }
+// The calling thread's last error message per context (null: no context).
+thread_local std::vector<std::pair<const xm_ctx*, std::string>> t_errors;
+
+std::string& error_slot(const xm_ctx* ctx) {
+    for (auto& e : t_errors)
+        if (e.first == ctx) return e.second;
+    t_errors.push_back({ctx, "no error"});
+    return t_errors.back().second;
 }
 
 void sync(xm_ctx* ctx) { XM_HIP(hipStreamSynchronize(ctx->stream)); }
@@ -112,6 +90,23 @@ uint64_t seg_max_chunks(const SegDesc& s, uint32_t rows) {
     return s.n ? uint64_t(s.n) / rows + uint64_t(s.zmax - s.zmin + 1) : 0;
 }
 
+}  // namespace
+
+namespace xm {
+
+namespace {
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    size_t p = 256;
+    while (p * 2 < bytes) p *= 2;  // p < bytes <= 2p
+    for (int q = 1; q <= 4; q++) {
+        const size_t c = p + (p / 4) * size_t(q);
+        if (c >= bytes) return c;
+    }
+    return 2 * p;
+}
+}  // namespace
+
 PairParams make_params(const xm_ctx* ctx, const SegDesc& A, const SegDesc& B, double radius_deg) {
     PairParams pp;
     pp.theta_deg = radius_deg;
@@ -146,22 +141,10 @@ void check_radius(double radius_deg, double h_a, double h_b) {
     if (std::ceil(radius_deg / h_a) > 1e6) throw InvalidArg{"radius / zone height is too large"};
 }
 
-}  // namespace
-
-namespace xm {
-
-namespace {
-size_t size_class(size_t bytes) {
-    if (bytes <= 256) return 256;
-    size_t p = 256;
-    while (p * 2 < bytes) p *= 2;  // p < bytes <= 2p
-    for (int q = 1; q <= 4; q++) {
-        const size_t c = p + (p / 4) * size_t(q);
-        if (c >= bytes) return c;
-    }
-    return 2 * p;
+void set_error(const xm_ctx* ctx, const std::string& msg) {
+    error_slot(ctx) = msg;
+    if (ctx) error_slot(nullptr) = msg;
 }
-}  // namespace
 
 // The pair kernel's shape when no tuned shape applies: the context's knobs.
 JobShape default_shape(const xm_ctx* ctx) {
@@ -622,9 +605,7 @@ void xm_ctx_destroy(xm_ctx* ctx) {
     delete ctx;
 }
 
-const char* xm_last_error_message(const xm_ctx* ctx) {
-    return ctx ? ctx->last_error.c_str() : g_tls_error.c_str();
-}
+const char* xm_last_error_message(const xm_ctx* ctx) { return error_slot(ctx).c_str(); }
 
 xm_status xm_ctx_set_timing(xm_ctx* ctx, int enable) {
     if (!ctx) return XM_EINVAL;
@@ -1272,11 +1253,11 @@ xm_status xm_match(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, double
     if (!ctx || !out) return XM_EINVAL;
     *out = nullptr;
     if (!a || !b) {
-        ctx->last_error = "null catalog";
+        set_error(ctx, "null catalog");
         return XM_EINVAL;
     }
     if (a->seg.size() != 1 || b->seg.size() != 1) {
-        ctx->last_error = "xm_match joins one-segment catalogs; use xm_match_batch";
+        set_error(ctx, "xm_match joins one-segment catalogs; use xm_match_batch");
         return XM_EINVAL;
     }
     const xm_job job{0, 0, radius_deg};

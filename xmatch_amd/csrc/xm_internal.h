@@ -39,13 +39,17 @@ struct TimedLaunch {
     hipEvent_t start, stop;
 };
 
+// Records msg as the calling thread's last error on ctx (null: with no context);
+// xm_last_error_message returns it.  Per thread, so that threads sharing a
+// context never read or overwrite each other's messages.
+void set_error(const xm_ctx* ctx, const std::string& msg);
+
 }  // namespace xm
 
 struct xm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    std::string last_error = "no error";
     bool timing = false;
     std::string timing_only;                 // time only this kernel (empty: all)
     std::vector<xm::TimedLaunch> launches;   // pending, resolved on query
@@ -216,6 +220,32 @@ class ScopedBlocks {
     std::vector<void*> held_;
 };
 
+// Runs f with the context's device current and turns the exceptions of the
+// library into a status (its message recorded by set_error).
+template <typename F>
+xm_status guarded(xm_ctx* ctx, F&& f) {
+    try {
+        if (ctx) XM_HIP(hipSetDevice(ctx->device));
+        f();
+        return XM_OK;
+    } catch (const InvalidArg& e) {
+        set_error(ctx, e.what);
+        return XM_EINVAL;
+    } catch (const OutOfMemory& e) {
+        set_error(ctx, e.what);
+        return XM_ENOMEM;
+    } catch (const DeviceError& e) {
+        set_error(ctx, e.what);
+        return e.code == hipErrorOutOfMemory ? XM_ENOMEM : XM_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return XM_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(ctx, e.what());
+        return XM_EINTERNAL;
+    }
+}
+
 // ---- launch bookkeeping: events around a kernel when timing is on
 void timing_begin(xm_ctx* ctx, const char* name, hipEvent_t* ev);
 void timing_end(xm_ctx* ctx, const char* name, hipEvent_t ev);
@@ -275,6 +305,8 @@ uint32_t* sort_error_word(xm_ctx* ctx);
 void sort_check(xm_ctx* ctx);
 // byte offset of the sort error word's host copy in the context's pinned scratch
 constexpr size_t PIN_SORT_ERR_OFFSET = 1024;
+// byte offset of the stage for small job tables (and other small uploads)
+constexpr size_t PIN_JOB_OFFSET = 4096;
 
 // ---- catalog kernels (xm_catalog.hip)
 // One-segment build: packed sort words (pack_word) of the records, their zone
@@ -401,6 +433,12 @@ JobShape job_shape(const xm_ctx* ctx, const SegDesc& a, const SegDesc& b, double
 // Tuning key of a job: log2 rows per occupied zone of each segment (rounded,
 // 0..31) and nz = ceil(r / h) (1..255): da | db << 8 | nz << 16.
 uint32_t tune_key(const SegDesc& a, const SegDesc& b, double radius_deg);
+
+// Join parameters of segment A with segment B at radius_deg (FP32 filter bounds
+// included); check_radius rejects radii outside (0, 90) degrees, unequal zone
+// heights and more than 1e6 zones of search.
+PairParams make_params(const xm_ctx* ctx, const SegDesc& A, const SegDesc& B, double radius_deg);
+void check_radius(double radius_deg, double h_a, double h_b);
 
 // Per job: a_chunk_base = cstart[a_zoff_lo], waves[j] = its chunk count.
 void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t* cstart,

@@ -228,7 +228,7 @@ xm_status xm_server_create(xm_ctx* ctx, const xm_catalog* resident, const xm_ser
     *out = nullptr;
     uint64_t nseg = 0;
     if (xm_catalog_segment_count(resident, &nseg) != XM_OK || nseg != 1) {
-        ctx->last_error = "the resident catalog must have one segment";
+        xm::set_error(ctx, "the resident catalog must have one segment");
         return XM_EINVAL;
     }
     xm_server* s = new (std::nothrow) xm_server();
@@ -241,7 +241,7 @@ xm_status xm_server_create(xm_ctx* ctx, const xm_catalog* resident, const xm_ser
     if (!s->cfg.max_batch_requests) s->cfg.max_batch_requests = 4096;
     if (s->cfg.precision != XM_FP64_PLAIN && s->cfg.precision != XM_FP32_FILTER_FP64_RECHECK) {
         delete s;
-        ctx->last_error = "unknown precision mode";
+        xm::set_error(ctx, "unknown precision mode");
         return XM_EINVAL;
     }
     try {

@@ -220,7 +220,7 @@ extern "C" xm_status xm_synth_fresh(xm_ctx* ctx, const xm_shape* shape, uint64_t
         XM_HIP(hipStreamSynchronize(ctx->stream));
         return XM_OK;
     } catch (const DeviceError& e) {
-        ctx->last_error = e.what;
+        xm::set_error(ctx, e.what);
         return XM_EDEVICE;
     }
 }
@@ -244,7 +244,7 @@ extern "C" xm_status xm_synth_copies(xm_ctx* ctx, const xm_obj* d_a, uint64_t na
         XM_HIP(hipStreamSynchronize(ctx->stream));
         return XM_OK;
     } catch (const DeviceError& e) {
-        ctx->last_error = e.what;
+        xm::set_error(ctx, e.what);
         return XM_EDEVICE;
     }
 }

@@ -78,7 +78,7 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
         const double radius_deg = job->radius_deg;
         if (job->seg_a >= a->seg.size() || job->seg_b >= b->seg.size() || !(radius_deg > 0.0) ||
             a->seg[job->seg_a].h_deg != b->seg[job->seg_b].h_deg) {
-            ctx->last_error = "bad segment, bad radius or zone heights differ";
+            xm::set_error(ctx, "bad segment, bad radius or zone heights differ");
             return XM_EINVAL;
         }
         HostSeg A, B;
@@ -210,10 +210,10 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
         (void)full;
         return XM_OK;
     } catch (const DeviceError& e) {
-        ctx->last_error = e.what;
+        xm::set_error(ctx, e.what);
         return XM_EDEVICE;
     } catch (const std::exception& e) {
-        ctx->last_error = e.what();
+        xm::set_error(ctx, e.what());
         return XM_EINTERNAL;
     }
 }
