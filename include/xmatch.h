@@ -214,6 +214,25 @@ xm_status xm_match(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b, double
 xm_status xm_match_batch(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
                          const xm_job *jobs, uint64_t n_jobs, xm_precision precision,
                          uint64_t capacity_hint, xm_result **out, xm_match_stats *stats);
+/* xm_match without device allocation: the pairs of a join of two one-segment
+ * catalogs go straight into the caller's device arrays d_id_a, d_id_b and
+ * d_dist_arcsec, each of `capacity` elements, in no particular order; *n_matches
+ * receives the number of pairs.  No counterpart in the reference: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.
+ * Working memory comes from a file-scope device array of 64 MiB per GPU, held
+ * until the process ends; A's chunk table must fit in it (about 17 MB for 2e8 rows
+ * at 1"): a join that needs more returns XM_EINVAL with the sizes.  A join of more
+ * pairs than capacity returns XM_ENOMEM with the count in *n_matches and the
+ * arrays unwritten.  The FP32 filter needs catalogs built with the parameter
+ * fp32_rows = 1.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (two threads, two contexts of one device) race, and the caller
+ * serialises them.  Not timed by xm_ctx_set_timing; stats (may be NULL) carries
+ * n_matches, n_chunks, band_hits and max_dist_err_arcsec, join_ms is 0. */
+xm_status xm_match_noalloc(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
+                           double radius_deg, xm_precision precision, int64_t *d_id_a,
+                           int64_t *d_id_b, double *d_dist_arcsec, uint64_t capacity,
+                           uint64_t *n_matches, xm_match_stats *stats);
 xm_status xm_result_count(const xm_result *res, uint64_t *n);
 /* Device pointers to the n result triples (valid until xm_result_destroy). */
 xm_status xm_result_device_ptrs(const xm_result *res, const int64_t **id_a,

@@ -44,7 +44,7 @@ EXPORTED = [
     "xm_result_jobs", "xm_result_job_device_ptr", "xm_work_counts_job",
     "xm_server_create", "xm_server_submit", "xm_server_poll", "xm_server_fetch",
     "xm_server_error", "xm_server_get_stats", "xm_server_destroy",
-    "xm_synth_fresh", "xm_synth_copies", "xm_ctx_tune_table",
+    "xm_synth_fresh", "xm_synth_copies", "xm_ctx_tune_table", "xm_match_noalloc",
 ]
 
 
@@ -187,6 +187,8 @@ def lib():
             "xm_catalog_export": ([vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "xm_match": ([vp, vp, vp, dbl, ctypes.c_int, u64, pvp, ctypes.POINTER(MatchStats)],
                          ctypes.c_int),
+            "xm_match_noalloc": ([vp, vp, vp, dbl, ctypes.c_int, vp, vp, vp, u64,
+                                  ctypes.POINTER(u64), ctypes.POINTER(MatchStats)], ctypes.c_int),
             "xm_result_count": ([vp, ctypes.POINTER(u64)], ctypes.c_int),
             "xm_result_device_ptrs": ([vp, pvp, pvp, pvp], ctypes.c_int),
             "xm_result_copy": ([vp, vp, vp, vp, vp], ctypes.c_int),
@@ -367,6 +369,23 @@ class Context:
         self._check(lib().xm_match(self._h, a.handle, b.handle, radius_deg, precision,
                                    capacity_hint, ctypes.byref(r), ctypes.byref(stats)))
         return Result(self, r, stats)
+
+    def match_noalloc(self, a, b, radius_deg, id_a, id_b, dist, capacity,
+                      precision=FP64_PLAIN):
+        """xm_match_noalloc: the join's pairs into the caller's device buffers id_a,
+        id_b (int64) and dist (float64, arcsec) of `capacity` elements each; returns
+        (pair count, MatchStats).  Raises XmatchError (XM_ENOMEM) when the pairs
+        outnumber capacity; the count is then in the exception's `n_matches`."""
+        n = ctypes.c_uint64()
+        stats = MatchStats()
+        st = lib().xm_match_noalloc(self._h, a.handle, b.handle, radius_deg, precision,
+                                    id_a.ptr, id_b.ptr, dist.ptr, int(capacity),
+                                    ctypes.byref(n), ctypes.byref(stats))
+        if st != XM_OK:
+            err = XmatchError(st, lib().xm_last_error_message(self._h).decode())
+            err.n_matches = n.value
+            raise err
+        return n.value, stats
 
     def match_batch(self, a, b, jobs, precision=FP64_PLAIN, capacity_hint=0):
         """One launch over jobs [(seg_a, seg_b, radius_deg), ...]; the result carries

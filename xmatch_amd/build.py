@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libxmatch_amd.so")
 SOURCES = ["xm_capi.hip", "xm_sort.hip", "xm_catalog.hip", "xm_pair.hip", "xm_workcount.hip",
-           "xm_synth.hip", "xm_server.hip"]
+           "xm_synth.hip", "xm_server.hip", "xm_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

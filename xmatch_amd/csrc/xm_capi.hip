@@ -579,6 +579,7 @@ xm_status xm_ctx_create(int device, xm_ctx** out) {
         XM_HIP(hipSetDevice(device));
         XM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         XM_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
+        ctx->noalloc_scratch = noalloc_scratch(device);
     });
     if (st != XM_OK) {
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1016,7 +1017,7 @@ void tune_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
                     for (int rep = 0; rep < 2; rep++) {
                         launch_fill_u64(ctx, counters, size_t(ro.n_regions + P) * REGION_STRIDE, 0);
                         XM_HIP(hipEventRecord(e0, ctx->stream));
-                        launch_pair(ctx, A, B, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
+                        launch_pair(ctx, A, B, static_cast<const Chunk*>(A->chunks), jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
                         XM_HIP(hipEventRecord(e1, ctx->stream));
                         XM_HIP(hipEventSynchronize(e1));
                         float ms = 0.f;
@@ -1165,7 +1166,7 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
             }
             launch_fill_u64(ctx, ro.counters, size_t(R + P) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev[0], ctx->stream));
-            launch_pair(ctx, A, b, jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
+            launch_pair(ctx, A, b, static_cast<const Chunk*>(A->chunks), jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
             launch_region_scan(ctx, ro, prefix, tail, next_base, next_cap);
             XM_HIP(hipEventRecord(ev[1], ctx->stream));
             XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,

@@ -39,6 +39,10 @@ struct TimedLaunch {
     hipEvent_t start, stop;
 };
 
+// The device address of the allocation-free join's scratch array on device
+// `device`, which must be current (hipGetSymbolAddress, once per device).
+void* noalloc_scratch(int device);
+
 // Records msg as the calling thread's last error on ctx (null: with no context);
 // xm_last_error_message returns it.  Per thread, so that threads sharing a
 // context never read or overwrite each other's messages.
@@ -57,6 +61,8 @@ struct xm_ctx {
     std::vector<std::pair<std::string, std::pair<double, uint64_t>>> kernel_totals;
     // pinned host scratch for small device->host reads
     void* pinned = nullptr;
+    // device address of the allocation-free join's scratch array on this device
+    void* noalloc_scratch = nullptr;
     // radix sort error word (device), set by a look-back that hit its polling limit
     uint32_t* sort_err = nullptr;
     // device memory cache: blocks keyed by size class; reused in stream order
@@ -263,6 +269,10 @@ inline void launch_timed(xm_ctx* ctx, const char* name, F&& f) {
 // ---- scan / sort (xm_sort.hip)
 // Exclusive prefix sum of n uint32 values in place.
 void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
+// The same with caller scratch for the tile sums: scan_partial_words(n) words.
+constexpr uint64_t SCAN_TILE = 4096;
+inline uint64_t scan_partial_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+void scan_exclusive_u32_with(xm_ctx* ctx, uint32_t* data, uint64_t n, uint32_t* partial);
 // Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), in 8-bit digits
 // from lo_bit up; bits from hi_bit up must be equal in all keys.  Buffers are
 // ping-ponged; on return *keys points at the sorted keys and *keys_alt at scratch.
@@ -398,6 +408,7 @@ struct RegionOut {
     // wide slots hold (idA, idB, chord^2), written while the rows are in cache;
     // narrow slots hold (A row, B row), 8 bytes instead of 24, for very large outputs
     bool wide;
+    bool dist_out;           // wide slots get the distance in arcsec instead of chord^2
     int64_t* id_a;           // wide: [slots]
     int64_t* id_b;
     double* d2;
@@ -448,9 +459,9 @@ uint32_t pair_blocks(uint64_t n_waves);
 // job_prefix[j] <= g < job_prefix[j+1] (job_prefix[n_jobs] = total, on the
 // device); the grid covers max_waves.  f32 = FP32 filter + FP64 recheck (needs
 // row32 of both catalogs), else FP64 plain.
-void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const JobDesc* jobs,
-                 const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
-                 const RegionOut& ro);
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Chunk* chunks,
+                 const JobDesc* jobs, const uint32_t* job_prefix, uint32_t n_jobs,
+                 uint64_t max_waves, bool f32, const RegionOut& ro);
 void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
 // Regions of cap_each slots back to back, then SPILL_POOLS pools of pool_cap.
 void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t pool_cap,

@@ -136,6 +136,7 @@ struct Out {
     uint64_t base, cap;                        // this block's region slots
     uint64_t spill_base, spill_cap;
     bool wide;
+    bool dist_out;                             // wide: write arcsec, not chord^2
     int64_t* __restrict__ id_a;                // wide slot arrays
     int64_t* __restrict__ id_b;
     double* __restrict__ d2;
@@ -164,6 +165,7 @@ __device__ inline Out make_out(const Row64* arow, const Row64* brow, const Regio
     o.spill_base = ro.base[pool];
     o.spill_cap = ro.spill_cap;
     o.wide = ro.wide;
+    o.dist_out = ro.dist_out;
     o.id_a = ro.id_a;
     o.id_b = ro.id_b;
     o.d2 = ro.d2;
@@ -242,7 +244,7 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
                 if (o.wide) {
                     o.id_a[slot] = o.arow[st.ra[e]].id;
                     o.id_b[slot] = o.brow[st.rb[e]].id;
-                    o.d2[slot] = st.d2[e];
+                    o.d2[slot] = o.dist_out ? dist_arcsec(st.d2[e]) : st.d2[e];
                 } else {
                     o.ra[slot] = st.ra[e];
                     o.rb[slot] = st.rb[e];
@@ -815,14 +817,13 @@ void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out) {
     });
 }
 
-void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const JobDesc* jobs,
-                 const uint32_t* job_prefix, uint32_t n_jobs, uint64_t max_waves, bool f32,
-                 const RegionOut& ro) {
+void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Chunk* chunks,
+                 const JobDesc* jobs, const uint32_t* job_prefix, uint32_t n_jobs,
+                 uint64_t max_waves, bool f32, const RegionOut& ro) {
     if (B->n == 0 || A->n == 0 || n_jobs == 0 || max_waves == 0) return;
     const dim3 grid(pair_blocks(max_waves));
     const Row64* arow = static_cast<const Row64*>(A->row);
     const Row64* brow = static_cast<const Row64*>(B->row);
-    const Chunk* chunks = static_cast<const Chunk*>(A->chunks);
     if (f32) {
         launch_timed(ctx, "pair_fp32", [&] {
             hipLaunchKernelGGL(k_pair<true>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,

@@ -425,7 +425,20 @@ void scatter_passes(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t 
 
 }  // namespace
 
+static_assert(SC_TILE == SCAN_TILE, "the scan's tile");
+
 void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
+    if (n == 0) return;
+    const uint64_t nt = (n + SC_TILE - 1) / SC_TILE;
+    if (nt == 1) {
+        scan_exclusive_u32_with(ctx, data, n, nullptr);
+        return;
+    }
+    ScopedBlocks tmp(ctx);
+    scan_exclusive_u32_with(ctx, data, n, tmp.alloc<uint32_t>(nt * sizeof(uint32_t)));
+}
+
+void scan_exclusive_u32_with(xm_ctx* ctx, uint32_t* data, uint64_t n, uint32_t* partial) {
     if (n == 0) return;
     const uint64_t nt = (n + SC_TILE - 1) / SC_TILE;
     if (nt == 1) {
@@ -436,8 +449,6 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n) {
         return;
     }
     if (nt > 0xffffffffull) throw InvalidArg{"scan too large"};
-    ScopedBlocks tmp(ctx);
-    uint32_t* partial = tmp.alloc<uint32_t>(nt * sizeof(uint32_t));
     launch_timed(ctx, "scan_reduce", [&] {
         hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(nt)), dim3(SC_THREADS), 0, ctx->stream,
                            data, n, partial);
