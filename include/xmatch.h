@@ -249,18 +249,27 @@ void xm_result_destroy(xm_result *res);
 /* ---- serving: a resident catalog answering crossmatch requests.  No counterpart
  * in the reference, which ran one batch of jobs and exited (BoostXmatch.cpp:
  * 301-330); built on xm_catalog_create_segments + xm_match_batch.
- * A server owns a worker thread that batches the pending requests continuously:
- * it takes every waiting request (up to the admission limits: query rows per
- * batch and free device memory), prepares them as one many-segment catalog, joins
- * each with the resident catalog in one batched launch, and hands each request
- * its own pairs.  A request's pairs do not depend on what it was batched with.
- * The resident catalog must outlive the server; requests use its zone height. */
+ * A server runs two threads, each on its own stream of the resident catalog's
+ * GPU (both of the default priority).  The preparing thread, on a context of its
+ * own, takes the waiting requests up to the admission limits (query rows and
+ * requests per batch, free device memory), uploads them and prepares them as one
+ * many-segment catalog.  The joining thread, on the resident catalog's context,
+ * joins every segment with the resident catalog in one batched launch and hands
+ * each request its pairs.  Batch k+1 is prepared while batch k joins.  A request
+ * of more than split_rows sources is cut into pieces of split_rows; a batch takes
+ * the next piece of each waiting request in turn, so a large request shares its
+ * batches with the requests behind it instead of holding them up.  A request's
+ * pairs do not depend on what it was batched with or how it was cut.  The
+ * resident catalog must outlive the server; requests use its zone height.  Other
+ * calls on the resident catalog's context wait for the server's joins (the
+ * context's mutex). */
 typedef struct xm_server xm_server;
 
 typedef struct {
-    uint64_t max_batch_rows;      /* query rows per batch (0 = 1 << 24) */
-    uint32_t max_batch_requests;  /* requests per batch (0 = 4096) */
+    uint64_t max_batch_rows;      /* query rows per batch (0 = 1 << 18) */
+    uint32_t max_batch_requests;  /* request pieces per batch (0 = 4096) */
     xm_precision precision;
+    uint64_t split_rows;          /* rows per piece of a large request (0 = 16384) */
 } xm_server_config;
 
 typedef struct {
@@ -268,7 +277,10 @@ typedef struct {
     uint64_t batches;
     uint64_t rows;               /* query rows joined */
     uint64_t matches;
-    double busy_ms;              /* wall time the worker spent on batches */
+    double busy_ms;              /* wall time the joining thread spent on batches */
+    double prep_ms;              /* wall time the preparing thread spent on batches */
+    double overlap_ms;           /* wall time both were busy at once */
+    uint64_t pieces;             /* request pieces joined */
 } xm_server_stats;
 
 xm_status xm_server_create(xm_ctx *ctx, const xm_catalog *resident, const xm_server_config *cfg,

@@ -2,6 +2,7 @@
 oracle's join of that request with the resident catalog, whatever it was batched
 with (alone = in a mix)."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -142,3 +143,70 @@ def test_destroy_with_queued_requests(gpu_ctx, resident):
     for i in range(20):
         srv.submit(make_query(rng, A, 1000, 0.05, id0=10**10 + 10**4 * i), 2.0 / 3600)
     srv.close()  # drains the queue, then stops
+
+
+def test_large_request_does_not_hold_up_small_ones(gpu_ctx, resident):
+    # a 1e6-row request cut into 8192-row pieces, batches of at most 32768 rows: a
+    # small request submitted just after it shares the next batches and finishes
+    # long before it, where a stop-and-wait server would finish the large request
+    # first; both equal the oracle
+    A, cat = resident
+    rng = np.random.default_rng(7)
+    big = make_query(rng, A, 1_000_000, 0.3, id0=2 * 10**10)
+    small = make_query(rng, A, 50, 0.02, id0=3 * 10**10)
+    r = 2.0 / 3600
+    with xm.Server(gpu_ctx, cat, max_batch_rows=32768, split_rows=8192) as srv:
+        t0 = time.perf_counter()
+        tb = srv.submit(big, r)
+        ts = srv.submit(small, r)
+        t1 = time.perf_counter()
+        rs = srv.result(ts)
+        rb = srv.result(tb)
+        st = srv.stats()
+    # the small one's completion (at most t1 + its latency) precedes the large
+    # one's (at least t0 + its latency) by a wide margin
+    assert t1 + rs[3] / 1e3 < t0 + rb[3] / 1e3
+    assert st["pieces"] >= 1_000_000 // 8192 + 1 and st["batches"] >= 30
+    check(rs, small, A, r)
+    check(rb, big, A, r)
+
+
+def test_alone_equals_pieces_in_mix(gpu_ctx, resident):
+    # a request cut into pieces and mixed with others gives the pairs it gets alone
+    A, cat = resident
+    rng = np.random.default_rng(8)
+    q = make_query(rng, A, 30_000, 0.1, id0=4 * 10**10)
+    r = 3.0 / 3600
+    with xm.Server(gpu_ctx, cat) as srv:
+        alone = srv.result(srv.submit(q, r))
+    with xm.Server(gpu_ctx, cat, max_batch_rows=20_000, split_rows=4096) as srv:
+        others = [srv.submit(make_query(rng, A, int(rng.integers(1, 5000)), 0.05,
+                                        id0=5 * 10**10 + 10**5 * i), 2.0 / 3600)
+                  for i in range(20)]
+        t = srv.submit(q, r)
+        mixed = srv.result(t)
+        for tk in others:
+            srv.result(tk)
+        st = srv.stats()
+    assert st["pieces"] > 20 + 30_000 // 4096
+    oa = np.lexsort((alone[1], alone[0]))
+    om = np.lexsort((mixed[1], mixed[0]))
+    for x, y in zip(alone[:3], mixed[:3]):
+        assert np.array_equal(x[oa], y[om])
+    check(mixed, q, A, r)
+
+
+def test_preparation_overlaps_joins(gpu_ctx, resident):
+    # a stream of requests keeps both threads busy: the next batch is prepared
+    # while the current one joins
+    A, cat = resident
+    rng = np.random.default_rng(9)
+    qs = [make_query(rng, A, int(rng.integers(2000, 20000)), 0.1, id0=6 * 10**10 + 10**5 * i)
+          for i in range(60)]
+    with xm.Server(gpu_ctx, cat, max_batch_rows=40_000) as srv:
+        tickets = [srv.submit(q, 2.0 / 3600) for q in qs]
+        for t in tickets:
+            srv.result(t)
+        st = srv.stats()
+    assert st["batches"] >= 10
+    assert st["prep_ms"] > 0 and st["busy_ms"] > 0 and st["overlap_ms"] > 0

@@ -116,13 +116,14 @@ class TuneEntry(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch_rows", ctypes.c_uint64), ("max_batch_requests", ctypes.c_uint32),
-                ("precision", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("split_rows", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("requests_done", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("rows", ctypes.c_uint64), ("matches", ctypes.c_uint64),
-                ("busy_ms", ctypes.c_double)]
+                ("busy_ms", ctypes.c_double), ("prep_ms", ctypes.c_double),
+                ("overlap_ms", ctypes.c_double), ("pieces", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -610,10 +611,10 @@ class Server:
     (id_query, id_catalog, dist_arcsec, latency_ms)."""
 
     def __init__(self, ctx, resident, max_batch_rows=0, max_batch_requests=0,
-                 precision=FP64_PLAIN):
+                 precision=FP64_PLAIN, split_rows=0):
         self.ctx = ctx
         self.resident = resident  # kept alive for the server's lifetime
-        cfg = ServerConfig(max_batch_rows, max_batch_requests, precision)
+        cfg = ServerConfig(max_batch_rows, max_batch_requests, precision, split_rows)
         self._h = ctypes.c_void_p()
         ctx._check(lib().xm_server_create(ctx.handle, resident.handle, ctypes.byref(cfg),
                                           ctypes.byref(self._h)))

@@ -1065,35 +1065,63 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     xm_match_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
+    // views of catalogs of another context, and the blocks made for them
+    xm_catalog a_view, b_view;
+    std::vector<void*> local;
     xm_status st = guarded(ctx, [&] {
-        if (a->ctx != ctx || b->ctx != ctx) throw InvalidArg{"catalog belongs to another context"};
+        if (a->ctx->device != ctx->device || b->ctx->device != ctx->device)
+            throw InvalidArg{"catalog of another device"};
         if (precision != XM_FP64_PLAIN && precision != XM_FP32_FILTER_FP64_RECHECK)
             throw InvalidArg{"unknown precision mode"};
         const bool f32 = precision == XM_FP32_FILTER_FP64_RECHECK;
         xm_catalog* A = const_cast<xm_catalog*>(a);
         xm_catalog* Bm = const_cast<xm_catalog*>(b);
+        // A catalog of another context on this device is read and never changed:
+        // what the join caches in a catalog of its own context -- A's chunk table
+        // (with its segment table's chunk rows) and FP32 rows -- is made in this
+        // context's memory for the join's duration.
+        if (a->ctx != ctx) {
+            a_view = *a;
+            a_view.ctx = ctx;
+            a_view.chunks = nullptr;
+            a_view.chunk_start = nullptr;
+            for (auto& d : a_view.seg) d.chunk_rows = 0;
+            local.reserve(local.size() + 1);
+            a_view.seg_dev = static_cast<SegDesc*>(dalloc(ctx, a_view.seg.size() * sizeof(SegDesc)));
+            local.push_back(a_view.seg_dev);
+            A = &a_view;
+        }
+        if (b->ctx != ctx) {
+            b_view = *b;
+            b_view.ctx = ctx;
+            Bm = &b_view;
+        }
         if (f32) {
             for (xm_catalog* c : {A, Bm}) {
                 if (c->row32 || c->n == 0) continue;
+                const bool view = c == &a_view || c == &b_view;
+                if (view) local.reserve(local.size() + 1);
                 c->row32 = dalloc(ctx, c->n * sizeof(Row32));
+                if (view) local.push_back(c->row32);
                 launch_to_row32(ctx, static_cast<const Row64*>(c->row), c->n,
                                 static_cast<Row32*>(c->row32));
             }
         }
-        if (ctx->autotune) tune_jobs(ctx, A, b, jobs, n_jobs, f32);
+        const xm_catalog* B = Bm;
+        if (ctx->autotune) tune_jobs(ctx, A, B, jobs, n_jobs, f32);
         std::vector<JobShape> shape(n_jobs);
         for (uint64_t j = 0; j < n_jobs; j++) {
-            if (jobs[j].seg_a >= A->seg.size() || jobs[j].seg_b >= b->seg.size())
+            if (jobs[j].seg_a >= A->seg.size() || jobs[j].seg_b >= B->seg.size())
                 throw InvalidArg{"job " + std::to_string(j) + " names a segment the catalog lacks"};
-            shape[j] = job_shape(ctx, A->seg[jobs[j].seg_a], b->seg[jobs[j].seg_b],
+            shape[j] = job_shape(ctx, A->seg[jobs[j].seg_a], B->seg[jobs[j].seg_b],
                                  jobs[j].radius_deg);
         }
-        make_jobs(ctx, A, b, jobs, n_jobs, shape, jt);
+        make_jobs(ctx, A, B, jobs, n_jobs, shape, jt);
         uint64_t cap = capacity_hint;
         if (!cap) {
             // the reference's default, 2 * max(nA, nB) per job (Worker.cu:295-300)
             for (uint64_t j = 0; j < n_jobs; j++)
-                cap += 2 * std::max<uint64_t>(A->seg[jobs[j].seg_a].n, b->seg[jobs[j].seg_b].n);
+                cap += 2 * std::max<uint64_t>(A->seg[jobs[j].seg_a].n, B->seg[jobs[j].seg_b].n);
         }
         // a default that free memory cannot hold is cut to a third of it (the
         // context's cached blocks count as free; slots narrow past the wide limit);
@@ -1166,7 +1194,7 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
             }
             launch_fill_u64(ctx, ro.counters, size_t(R + P) * REGION_STRIDE, 0);
             XM_HIP(hipEventRecord(ev[0], ctx->stream));
-            launch_pair(ctx, A, b, static_cast<const Chunk*>(A->chunks), jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
+            launch_pair(ctx, A, B, static_cast<const Chunk*>(A->chunks), jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
             launch_region_scan(ctx, ro, prefix, tail, next_base, next_cap);
             XM_HIP(hipEventRecord(ev[1], ctx->stream));
             XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -1204,7 +1232,7 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
         res->cap = slots_out;
         res->n = total;
         XM_HIP(hipEventRecord(ev[2], ctx->stream));
-        launch_compact(ctx, ro, prefix, total, A, b, res->id_a, res->id_b, res->dist, res->job);
+        launch_compact(ctx, ro, prefix, total, A, B, res->id_a, res->id_b, res->dist, res->job);
         XM_HIP(hipEventRecord(ev[3], ctx->stream));
         if (batch) {
             launch_job_sum(ctx, ro.job_count, uint32_t(n_jobs), job_sums);
@@ -1221,6 +1249,9 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
     });
     (void)guarded(ctx, [&] {
         sync(ctx);
+        for (void* p : local) dfree(ctx, p);
+        dfree(ctx, a_view.chunks);
+        dfree(ctx, a_view.chunk_start);
         dfree(ctx, ro.counters);
         dfree(ctx, scratch);
         free_slots(ctx, ro);
