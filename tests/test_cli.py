@@ -75,10 +75,20 @@ def test_selfjoin_output_matches_golden(tmp_path, r_arcsec):
     assert r.stdout.startswith("%d matches" % len(g["ida"]))
 
 
+def manifest_plan(outdir):
+    """(bands, round rows) of the plan in <outdir>/manifest."""
+    lines = open(os.path.join(outdir, "manifest")).read().splitlines()
+    plan = [ln for ln in lines if ln.startswith("plan ")][0].split()
+    kv = dict(x.split("=") for x in plan[1:])
+    bands = [tuple(int(v) for v in ln.split()[1:]) for ln in lines if ln.startswith("band ")]
+    assert len(bands) == int(kv["bands"])
+    return bands, int(kv["round_rows"])
+
+
 @pytest.mark.gpu
 def test_segments_rounds_and_reference_format(tmp_path):
-    # 2500-object segments, B in rounds of 5000 records: 2 rounds of 4 x 2 jobs,
-    # written as 16-byte {idA, idB} records to <outdir>/0 and <outdir>/1
+    # -n 2500: A cut into declination bands of at most 2500 records, each one
+    # segment; B streamed in rounds of 5000 records; 16-byte {idA, idB} records
     out = tmp_path / "out"
     out.mkdir()
     (out / "stale").write_bytes(b"x")
@@ -88,8 +98,33 @@ def test_segments_rounds_and_reference_format(tmp_path):
     assert not (out / "stale").exists()
     g = np.load(os.path.join(GOLD, "selfjoin_r2.npz"))
     x, nfiles = read_pairs(out, fmt=16)
-    assert nfiles == 2
+    bands, rows = manifest_plan(out)
+    assert len(bands) >= 4 and rows == 5000
+    assert all(b[2] <= 2500 for b in bands) and sum(b[2] for b in bands) == 10_000
+    assert nfiles >= len(bands)
     assert np.array_equal(x["a"], g["ida"]) and np.array_equal(x["b"], g["idb"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_declination_bands_and_rounds_equal_the_golden(tmp_path, precision):
+    # A out of core: bands of at most 3000 records (at least 3), B streamed past
+    # each band in rounds of at most 1500 records (at least 2 in a band); every
+    # A row in one band, so the union of the round files is the self-join once
+    out = tmp_path / "out"
+    r = run(["-o", str(out), "-r", "2", "-a", "3000", "-b", "1500", "-p", precision, "-v",
+             SAMPLE, SAMPLE])
+    assert r.returncode == 0, r.stdout + r.stderr
+    bands, rows = manifest_plan(out)
+    assert len(bands) >= 3 and rows == 1500
+    assert all(b[2] <= 3000 for b in bands) and sum(b[2] for b in bands) == 10_000
+    # declination bands: disjoint zone ranges in order
+    assert all(bands[i][1] < bands[i + 1][0] for i in range(len(bands) - 1))
+    x, nfiles = read_pairs(out)
+    assert nfiles >= len(bands) + 1    # some band took two rounds or more
+    g = np.load(os.path.join(GOLD, "selfjoin_r2.npz"))
+    assert np.array_equal(x["a"], g["ida"]) and np.array_equal(x["b"], g["idb"])
+    assert np.abs(x["d"] - g["dist_arcsec"]).max() <= 1e-6
 
 
 @pytest.mark.gpu
@@ -106,10 +141,59 @@ def test_resume_skips_finished_rounds(tmp_path):
     r = run(args)
     assert r.returncode == 0, r.stdout + r.stderr
     full, nfiles = read_pairs(out)
-    assert nfiles == 4
+    assert nfiles >= 4
     os.remove(out / "2")                   # as if the run had stopped in round 2
     (out / "2.part").write_bytes(b"partial")
     r = run(args[:2] + ["-c"] + args[2:])
-    assert r.returncode == 0 and "3 round(s) done before" in r.stdout
-    again, nfiles = read_pairs(out)
-    assert nfiles == 4 and np.array_equal(again, full)
+    assert r.returncode == 0 and "%d round(s) done before" % (nfiles - 1) in r.stdout
+    again, n2 = read_pairs(out)
+    assert n2 == nfiles and np.array_equal(again, full)
+
+
+def write_manifest_of(tmp_path, args):
+    """An output directory whose manifest another run wrote: the first lines of a
+    manifest name the inputs and the options."""
+    out = tmp_path / "out"
+    out.mkdir()
+    ident = ["xmatch-manifest 1"]
+    for tag, path in (("a", SAMPLE), ("b", SAMPLE)):
+        st = os.stat(path)
+        ident.append("%s %d %d.%d %s" % (tag, st.st_size, st.st_mtime_ns // 10**9,
+                                         st.st_mtime_ns % 10**9, path))
+    ident.append(args)
+    (out / "manifest").write_text("\n".join(ident) + "\nplan round_rows=5000 bands=1\n"
+                                  "band 0 648000 10000\n")
+    return out
+
+
+def test_resume_refuses_other_options(tmp_path):
+    out = write_manifest_of(tmp_path, "options r=2 z=2 n=0 f=24 p=fp64 a=0 b=0")
+    r = run(["-o", str(out), "-c", "-r", "5", SAMPLE, SAMPLE])
+    assert r.returncode == 1 and "other inputs or options" in r.stdout + r.stderr
+    r = run(["-o", str(out), "-c", "-r", "2", "-p", "fp32", SAMPLE, SAMPLE])
+    assert r.returncode == 1 and "other inputs or options" in r.stdout + r.stderr
+
+
+def test_resume_refuses_other_inputs(tmp_path):
+    out = write_manifest_of(tmp_path, "options r=2 z=2 n=0 f=24 p=fp64 a=0 b=0")
+    other = tmp_path / "other.idradec"
+    other.write_bytes(open(SAMPLE, "rb").read())
+    r = run(["-o", str(out), "-c", "-r", "2", SAMPLE, str(other)])
+    assert r.returncode == 1 and "other inputs or options" in r.stdout + r.stderr
+
+
+def test_resume_without_manifest_is_refused(tmp_path):
+    out = tmp_path / "out"
+    out.mkdir()
+    (out / "0").write_bytes(b"")
+    r = run(["-o", str(out), "-c", "-r", "2", SAMPLE, SAMPLE])
+    assert r.returncode == 1 and "no manifest" in r.stdout + r.stderr
+
+
+def test_resume_with_the_same_options_passes_the_check(tmp_path):
+    # the same inputs and options: the run goes on to the GPU (on a machine
+    # without one it stops there, after the check)
+    out = write_manifest_of(tmp_path, "options r=2 z=2 n=0 f=24 p=fp64 a=0 b=0")
+    r = run(["-o", str(out), "-c", "-r", "2", SAMPLE, SAMPLE])
+    assert r.returncode == 0 or (r.returncode == 4 and "no usable GPU" in r.stderr), \
+        r.stdout + r.stderr
