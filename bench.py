@@ -287,7 +287,7 @@ def run_headline(args):
     dom = None
     # the FP32 form first: its dominant kernel is the one whose launches the timed
     # steps of both forms carry events around
-    for name, prec in MODES[::-1]:
+    for name, prec in [m for m in MODES[::-1] if args.mode in ("both", m[0].split("_")[0])]:
         ctx.set_param("fp32_rows", 1 if prec == xm.FP32_FILTER_FP64_RECHECK else 0)
         step = join_step(ctx, dA, dB, r, prec)
         ms, table, only, (dtot, dcnt), (nm, st) = timed_steps(ctx, step, args.warmup,
@@ -313,6 +313,12 @@ def run_headline(args):
         log("%s: %.2f ms per step (join %.2f ms), %d matches"
             % (name, ms, st["join_ms"], nm))
     ctx.set_param("fp32_rows", 0)
+    if args.mode != "both":
+        # one form alone (for counter runs that must not mix the forms' kernels)
+        print(json.dumps({"mode": args.mode, "modes": modes, "w_cand": w["w_cand"],
+                          "w_tile": w["w_tile"], "digests": digests}), flush=True)
+        ctx.close()
+        return
     f32, f64 = modes["fp32_filter_fp64_recheck"], modes["fp64_plain"]
     # the dominant kernel of the FP32 form: its algorithmic bytes per launch (the
     # gather writes the FP32 rows too) over its average launch in the timed steps
@@ -390,6 +396,8 @@ def run_full_sky(args):
         A = A[np.argsort(A["dec"], kind="stable")]
         B = B[np.argsort(B["dec"], kind="stable")]
     ctx = xm.Context(0)
+    if args.chunk_rows:
+        ctx.set_param("chunk_rows", args.chunk_rows)
     dA = ctx.to_device(A)
     dB = ctx.to_device(B)
     ms, table, dom, (dtot, dcnt), (nm, st) = timed_steps(
@@ -472,12 +480,38 @@ def run_batch(args):
     jobs = xm.job_table([(j, j, rs[j]) for j in range(n_jobs)])
 
     prec = xm.FP32_FILTER_FP64_RECHECK if args.precision == "fp32" else xm.FP64_PLAIN
+    if args.chunk_rows:
+        ctx.set_param("chunk_rows", args.chunk_rows)
 
     def batched():
         A = ctx.catalog_segments(seg_a)
         B = ctx.catalog_segments(seg_b)
         res = ctx.match_batch(A, B, jobs, precision=prec)
         return A, B, res
+
+    # the FP32 form's catalogs carry their FP32 rows from the gather
+    ctx.set_param("fp32_rows", 1 if prec == xm.FP32_FILTER_FP64_RECHECK else 0)
+    fixed_ms = None
+    if args.autotune:
+        # the fixed default shape first, then the tuned shapes (the warmup calls
+        # measure every tuning key the batch holds)
+        for _ in range(args.warmup):
+            A, B, res = batched()
+            res.close(); A.close(); B.close()
+        ctx.synchronize()
+        fm = []
+        for i in range(args.steps):
+            t = time.perf_counter()
+            A, B, res = batched()
+            fm.append(1e3 * (time.perf_counter() - t))
+            res.close(); A.close(); B.close()
+        fixed_ms = float(np.median(fm))
+        log("fixed default shape: %.1f ms per step" % fixed_ms)
+        ctx.set_param("autotune", 1)
+        t0 = time.time()
+        A, B, res = batched()
+        res.close(); A.close(); B.close()
+        log("tuning: %.1f s, %d keys" % (time.time() - t0, len(ctx.tune_table())))
 
     ctx.set_timing(True)
     for _ in range(args.warmup):
@@ -568,8 +602,15 @@ def run_batch(args):
         "per_job_counts_equal": bool(np.array_equal(counts_batch, counts_loop)),
         "sample_jobs_identical": sample_ok, "sample_jobs": len(sample),
         "kernels": kernels,
+        "shapes": {"autotune": bool(args.autotune), "fixed_default_ms": fixed_ms,
+                   "tuned_over_fixed": (fixed_ms / t_batch) if fixed_ms else None,
+                   "default": {"chunk_rows": ctx.get_param("chunk_rows"),
+                               "dense_min": ctx.get_param("dense_min")},
+                   "tune_table": ctx.tune_table()},
     }
     out.update(w)
+    ctx.set_param("autotune", 0)
+    ctx.set_param("fp32_rows", 0)
     dA.free(); dB.free()
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -686,6 +727,9 @@ def run_serve(args):
             "matches": int(nm.sum()), "matches_per_s": float(nm.sum() / wall),
             "batches": batches, "mean_requests_per_batch": n_q / max(batches, 1),
             "worker_busy_frac": (s1["busy_ms"] - s0["busy_ms"]) / 1e3 / wall,
+            "prep_busy_frac": (s1["prep_ms"] - s0["prep_ms"]) / 1e3 / wall,
+            "overlap_frac": (s1["overlap_ms"] - s0["overlap_ms"]) / 1e3 / wall,
+            "pieces": s1["pieces"] - s0["pieces"],
         })
         log("rate %.0f q/s: p50 %.2f ms, p99 %.2f ms (N <= 1e4: %.2f ms), %d batches"
             % (rate, per_rate[-1]["p50_ms"], per_rate[-1]["p99_ms"],
@@ -730,6 +774,8 @@ def main():
     ap.add_argument("--input-order", choices=["random", "dec"], default="random",
                     help="full_sky config: record order of both catalogs (dec = sorted by "
                          "declination; a diagnostic form)")
+    ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both",
+                    help="plane config: both precision forms (the headline line) or one alone")
     ap.add_argument("--no-full-sky", action="store_true",
                     help="plane config: skip the 50M x 50M FP64 figures reported beside it")
     ap.add_argument("--chunk-rows", type=int, default=0,

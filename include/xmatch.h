@@ -124,13 +124,14 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
 xm_status xm_ctx_set_param(xm_ctx *ctx, const char *name, int64_t value);
 xm_status xm_ctx_get_param(xm_ctx *ctx, const char *name, int64_t *value);
 /* The pair kernel's shape per job: "chunk_rows" (A rows per wavefront, a multiple
- * of 64 up to 1024; 0 = 512) and "dense_min" above are the shape every join uses,
+ * of 64 up to 512; 0 = 512) and "dense_min" above are the shape every join uses,
  * unless "autotune" is 1.  Then a join first looks up each job's tuning key --
  * log2 rows per occupied zone of its A and B segments (rounded) and nz =
  * ceil(r / h) -- in the context's tune table.  For a key the table lacks, it times
  * count-only pair launches of up to 16 of the batch's jobs with that key (the
- * largest) for chunk rows {64, 128, 256, 512, 1024} x dense_min {4, 16, 64} and
- * keeps the fastest shape.  The reference's kernel shape was fixed at compile time
+ * largest, repeated until the launch fills the GPU) for chunk rows {64, 128,
+ * 256, 512} x dense_min {4, 16, 64} and
+ * keeps the fastest shape if it beats the default shape by more than 3%.  The reference's kernel shape was fixed at compile time
  * (Worker.cu:22,351-353).  Results never depend on the shape. */
 typedef struct {
     uint32_t key;          /* log2 rows/zone of A | log2 rows/zone of B << 8 | nz << 16 */

@@ -60,7 +60,7 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A, const std::vector<uint32_t>& rows
     for (size_t s = 0; s < A->seg.size(); s++) {
         SegDesc& d = A->seg[s];
         if (rows[s] == 0 || rows[s] % WAVE || rows[s] > uint32_t(MAX_CHUNK_ROWS))
-            throw std::logic_error("chunk rows must be a multiple of 64 up to 1024");
+            throw std::logic_error("chunk rows must be a multiple of 64 up to 512");
         d.chunk_rows = rows[s];
         if (d.n) bound += uint64_t(d.n) / d.chunk_rows + uint64_t(d.zmax - d.zmin + 1);
     }
@@ -668,7 +668,7 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
             ctx->autotune = value != 0;
         } else if (n == "chunk_rows") {
             if (value < 0 || value > MAX_CHUNK_ROWS || value % WAVE)
-                throw InvalidArg{"chunk_rows must be 0 or a multiple of 64 up to 1024"};
+                throw InvalidArg{"chunk_rows must be 0 or a multiple of 64 up to 512"};
             ctx->chunk_rows = uint32_t(value);
         } else {
             throw InvalidArg{"unknown parameter '" + n + "'"};
@@ -950,10 +950,18 @@ void free_slots(xm_ctx* ctx, RegionOut& ro) {
 // Candidate shapes: chunk rows x dense rule.  Each is timed on a sample of the
 // batch's jobs of the key (up to TUNE_SAMPLE, the largest by nA * nB) as a
 // count-only pair launch: regions and spill pools of zero slots, so every hit is
-// counted and none is written.  The fastest of two launches per shape counts.
-constexpr uint32_t TUNE_ROWS[] = {64, 128, 256, 512, 1024};
+// counted and none is written.  The sample is repeated until the launch fills the
+// GPU.  The fastest of three launches per shape counts,
+// and a shape replaces the default only when it is faster by TUNE_MARGIN: on the
+// 10,000-job batch (165 keys) the sample launches of small keys are mostly launch
+// time, and choosing the fastest outright gave 249.6 ms against 247.0 ms with the
+// default shape everywhere.
+constexpr uint32_t TUNE_ROWS[] = {64, 128, 256, 512};
 constexpr uint32_t TUNE_DENSE[] = {4, 16, 64};
 constexpr size_t TUNE_SAMPLE = 16;
+constexpr uint64_t TUNE_WAVES = 16384;
+constexpr uint64_t TUNE_MAX_REPS = 256;
+constexpr double TUNE_MARGIN = 0.97;
 struct xm_tune_record {
     uint32_t rows, dense_min;
     double ms;
@@ -995,8 +1003,17 @@ void tune_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
             std::stable_sort(js.begin(), js.end(),
                              [&](uint64_t x, uint64_t y) { return work(x) > work(y); });
             if (js.size() > TUNE_SAMPLE) js.resize(TUNE_SAMPLE);
+            // the sample, repeated until one launch has TUNE_WAVES wavefronts (at the
+            // default rows): the launch then fills the GPU as the batch does, and its
+            // time measures a shape's throughput, not the latency of a few waves
+            uint64_t waves = 0;
+            for (uint64_t j : js)
+                waves += seg_max_chunks(A->seg[jobs[j].seg_a], default_shape(ctx).chunk_rows);
+            const uint64_t reps =
+                std::max<uint64_t>(1, std::min<uint64_t>(TUNE_MAX_REPS, TUNE_WAVES / std::max<uint64_t>(waves, 1)));
             std::vector<xm_job> sample;
-            for (uint64_t j : js) sample.push_back(jobs[j]);
+            for (uint64_t r = 0; r < reps; r++)
+                for (uint64_t j : js) sample.push_back(jobs[j]);
             xm_tune_record best{};
             bool have = false;
             double default_ms = -1.0;
@@ -1014,7 +1031,7 @@ void tune_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
                     ro.spill_cap = 0;
                     ro.wide = false;
                     float best_ms = 1e30f;
-                    for (int rep = 0; rep < 2; rep++) {
+                    for (int rep = 0; rep < 3; rep++) {
                         launch_fill_u64(ctx, counters, size_t(ro.n_regions + P) * REGION_STRIDE, 0);
                         XM_HIP(hipEventRecord(e0, ctx->stream));
                         launch_pair(ctx, A, B, static_cast<const Chunk*>(A->chunks), jt.jobs, jt.prefix, jt.n_jobs, jt.max_waves, f32, ro);
@@ -1034,7 +1051,10 @@ void tune_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
                     }
                 }
             }
-            xm_ctx::TuneEntry e{best.rows, best.dense_min, uint32_t(sample.size()), best.ms,
+            const JobShape def = default_shape(ctx);
+            if (default_ms >= 0.0 && best.ms > TUNE_MARGIN * default_ms)
+                best = xm_tune_record{def.chunk_rows, def.dense_min, default_ms};
+            xm_ctx::TuneEntry e{best.rows, best.dense_min, uint32_t(js.size()), best.ms,
                                 default_ms};
             ctx->tune.push_back({t.first, e});
         }
@@ -1126,17 +1146,28 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
         // a default that free memory cannot hold is cut to a third of it (the
         // context's cached blocks count as free; slots narrow past the wide limit);
         // a join that needs more re-runs once at its exact size
-        if (!capacity_hint) {
+        // free device memory, the context's cached blocks counted free
+        uint64_t avail = ~uint64_t(0);
+        {
             size_t free_b = 0, total_b = 0;
             if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-                uint64_t avail = free_b;
+                avail = free_b;
                 for (auto& fb : ctx->free_blocks) avail += fb.first;
-                const uint64_t job_b = batch ? 4 : 0;
-                const uint64_t slot_b = (cap * 24 <= WIDE_STAGING_BYTES ? 24 : 8) + job_b;
-                cap = std::min<uint64_t>(cap, avail / 3 / slot_b);
             } else {
                 (void)hipGetLastError();
             }
+        }
+        // wide slots while their staging stays under WIDE_STAGING_BYTES and a quarter
+        // of the free memory: the staging and the result are held together while
+        // the compaction runs (10,000 cap joins of 2.2e9 sources, FP32 form: 2.4e9
+        // default slots would take 67 GB wide against 29 GB narrow)
+        auto wide_fits = [&](uint64_t slots) {
+            return slots * 24 <= WIDE_STAGING_BYTES && slots * 24 <= avail / 4;
+        };
+        if (!capacity_hint && avail != ~uint64_t(0)) {
+            const uint64_t job_b = batch ? 4 : 0;
+            const uint64_t slot_b = (wide_fits(cap) ? 24 : 8) + job_b;
+            cap = std::min<uint64_t>(cap, avail / 3 / slot_b);
         }
         cap = std::max<uint64_t>(cap, 1024);
         for (auto& e : ev) XM_HIP(hipEventCreate(&e));
@@ -1178,7 +1209,7 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
             ro.base = base;
             ro.cap = capa;
             // wide slots unless the staging would pass WIDE_STAGING_BYTES
-            ro.wide = slots * 24 <= WIDE_STAGING_BYTES;
+            ro.wide = wide_fits(slots);
             if (ro.wide) {
                 ro.id_a = static_cast<int64_t*>(dalloc(ctx, slots * 8));
                 ro.id_b = static_cast<int64_t*>(dalloc(ctx, slots * 8));

@@ -149,7 +149,10 @@ constexpr int WAVE = 64;
 // average, up to ~505) took 7.09 ms in the pair kernel with 256-row chunks and
 // 5.93 ms with 512 (FP64, same box).
 constexpr int CHUNK_ROWS = 512;
-constexpr int MAX_CHUNK_ROWS = 1024;
+// the pair kernel stages a chunk's ra quanta in LDS (4 B a row per wavefront):
+// 1024-row chunks tied with 512 on every workload measured, and their 14 KB of
+// LDS per block held the kernel to 6 wavefronts per SIMD against 7
+constexpr int MAX_CHUNK_ROWS = 512;
 
 // One catalog row on the device: FP64 unit vector and id in one 32-byte line, so
 // a candidate B row is a single aligned fetch.
@@ -408,7 +411,6 @@ struct RegionOut {
     // wide slots hold (idA, idB, chord^2), written while the rows are in cache;
     // narrow slots hold (A row, B row), 8 bytes instead of 24, for very large outputs
     bool wide;
-    bool dist_out;           // wide slots get the distance in arcsec instead of chord^2
     int64_t* id_a;           // wide: [slots]
     int64_t* id_b;
     double* d2;
@@ -463,6 +465,8 @@ void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Ch
                  const JobDesc* jobs, const uint32_t* job_prefix, uint32_t n_jobs,
                  uint64_t max_waves, bool f32, const RegionOut& ro);
 void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
+// d[i] = 2 asin(sqrt(d[i]) / 2) in arcsec for i < n (chord^2 to distance, in place).
+void launch_chord_to_arcsec(xm_ctx* ctx, double* d, uint64_t n);
 // Regions of cap_each slots back to back, then SPILL_POOLS pools of pool_cap.
 void launch_region_init(xm_ctx* ctx, uint32_t n_regions, uint64_t cap_each, uint64_t pool_cap,
                         uint64_t* base, uint64_t* cap);

@@ -225,15 +225,16 @@ extern "C" xm_status xm_match_noalloc(xm_ctx* ctx, const xm_catalog* a, const xm
         if (total == 0) return;
 
         // run 2: each region's hits from its offset on, straight into the outputs
+        // (chord^2 in the distance array, turned into arcsec after the run)
         ro.base = next_base;
         ro.cap = next_cap;
-        ro.dist_out = true;
         ro.id_a = d_id_a;
         ro.id_b = d_id_b;
         ro.d2 = d_dist_arcsec;
         launch_fill_u64(ctx, counters, uint64_t(parts) * REGION_STRIDE, 0);
         launch_pair(ctx, a, b, chunks, job, jprefix, 1, n_chunks, f32, ro);
         launch_region_scan(ctx, ro, prefix, tail, base2, cap2);
+        launch_chord_to_arcsec(ctx, d_dist_arcsec, total);
         XM_HIP(hipMemcpyAsync(pin, tail, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         XM_HIP(hipStreamSynchronize(ctx->stream));
         if (pin[0] != total || pin[1] != 0)

@@ -2,7 +2,7 @@
 //
 // Replaces the per-zone-pair launch loop of Worker::Match and xmatch_kernel
 // (BoostXmatch/BoostXmatch/Worker.cu:330-359 and :65-241).  The A catalog is cut
-// into chunks of up to chunk_rows (64..1024, per A segment; 256 by default)
+// into chunks of up to chunk_rows (64..512, per A segment; 512 by default)
 // consecutive rows of one zone; each wavefront owns one
 // chunk and joins it with B zones z-nz..z+nz over the chunk's ra span widened by
 // alpha(zone) (Common.cu:16-23), the 0/360 seam split in two intervals.  Each
@@ -136,7 +136,6 @@ struct Out {
     uint64_t base, cap;                        // this block's region slots
     uint64_t spill_base, spill_cap;
     bool wide;
-    bool dist_out;                             // wide: write arcsec, not chord^2
     int64_t* __restrict__ id_a;                // wide slot arrays
     int64_t* __restrict__ id_b;
     double* __restrict__ d2;
@@ -165,7 +164,6 @@ __device__ inline Out make_out(const Row64* arow, const Row64* brow, const Regio
     o.spill_base = ro.base[pool];
     o.spill_cap = ro.spill_cap;
     o.wide = ro.wide;
-    o.dist_out = ro.dist_out;
     o.id_a = ro.id_a;
     o.id_b = ro.id_b;
     o.d2 = ro.d2;
@@ -244,7 +242,7 @@ __device__ inline void flush_stage(Stage& st, uint32_t lane, const Out& o) {
                 if (o.wide) {
                     o.id_a[slot] = o.arow[st.ra[e]].id;
                     o.id_b[slot] = o.brow[st.rb[e]].id;
-                    o.d2[slot] = o.dist_out ? dist_arcsec(st.d2[e]) : st.d2[e];
+                    o.d2[slot] = st.d2[e];
                 } else {
                     o.ra[slot] = st.ra[e];
                     o.rb[slot] = st.rb[e];
@@ -528,11 +526,20 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     uint32_t half = 1;
     while (2 * half < cnt) half *= 2;
 
-    // the chunk's ra quanta in LDS, padded with INT_MAX to 2 * half
-    for (uint32_t r0 = 0; r0 < 2 * half || r0 < cnt; r0 += WAVE) {
-        const uint32_t r = r0 + lane;
-        if (r < uint32_t(MAX_CHUNK_ROWS))
-            s_aq[w][r] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
+    // the chunk's ra quanta in LDS, padded with INT_MAX to 2 * half (>= cnt); the
+    // loads of four rows per lane are in flight together
+    for (uint32_t r0 = 0; r0 < 2 * half; r0 += 4 * WAVE) {
+        int32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t r = r0 + u * WAVE + lane;
+            v[u] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t r = r0 + u * WAVE + lane;
+            if (r < 2 * half) s_aq[w][r] = v[u];
+        }
     }
     wave_lds_sync();
     const uint64_t q_lo = uint32_t(s_aq[w][0]);
@@ -737,6 +744,12 @@ __global__ __launch_bounds__(256) void k_job_prep(JobDesc* __restrict__ jobs, ui
     waves[j] = jobs[j].b_zmin <= jobs[j].b_zmax ? c1 - c0 : 0u;  // empty B: no work
 }
 
+// chord^2 -> distance in arcsec, in place: the same conversion as k_compact's.
+__global__ __launch_bounds__(256) void k_chord_to_arcsec(double* __restrict__ d, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) d[i] = dist_arcsec(d[i]);
+}
+
 // FP64 row -> FP32 row (round to nearest), for the FP32 filter.
 __global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, uint64_t n,
                                                   Row32* __restrict__ out) {
@@ -806,6 +819,14 @@ void launch_region_scan(xm_ctx* ctx, const RegionOut& ro, uint64_t* prefix, uint
         hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1024), 0, ctx->stream,
                            (const uint64_t*)ro.counters, ro.n_regions, ro.cap, ro.spill_cap,
                            prefix, tail, next_base, next_cap);
+    });
+}
+
+void launch_chord_to_arcsec(xm_ctx* ctx, double* d, uint64_t n) {
+    if (n == 0) return;
+    launch_timed(ctx, "chord_to_arcsec", [&] {
+        hipLaunchKernelGGL(k_chord_to_arcsec, dim3(uint32_t((n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, d, n);
     });
 }
 
