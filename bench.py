@@ -270,6 +270,8 @@ def run_headline(args):
         ctx.set_param("chunk_rows", args.chunk_rows)
     if args.autotune:
         ctx.set_param("autotune", 1)
+    if args.key_ra_bits:
+        ctx.set_param("key_ra_bits", args.key_ra_bits)
     t0 = time.time()
     dA, dB = make_plane(ctx, n, PLANE_SEED)
     log("generated %d x %d plane-density sources on the device in %.2f s"
@@ -779,8 +781,10 @@ def main():
     ap.add_argument("--no-full-sky", action="store_true",
                     help="plane config: skip the 50M x 50M FP64 figures reported beside it")
     ap.add_argument("--chunk-rows", type=int, default=0,
-                    help="pair kernel: A rows per chunk (a multiple of 64 up to 1024; 0: the "
+                    help="pair kernel: A rows per chunk (a multiple of 64 up to 512; 0: the "
                          "library default)")
+    ap.add_argument("--key-ra-bits", type=int, default=0,
+                    help="catalog sort key: at most this many ra bits (0: the library's choice)")
     ap.add_argument("--autotune", action="store_true",
                     help="pair kernel: shapes picked by the library's tuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
