@@ -56,7 +56,7 @@ constexpr int PK_WAVES = 2;
 constexpr int PK_THREADS = PK_WAVES * WAVE;
 constexpr int STAGE = 128;  // two wavefronts of hits (50M x 50M: 64 -> 1.45 ms, 128 -> 1.38, 256 -> 1.44)
 constexpr int UNROLL = 4;
-constexpr uint32_t ONE_PASS_ROWS = 4 * WAVE;  // B zones up to this size are read in one pass
+constexpr uint32_t ONE_PASS_ROWS = 8 * WAVE;  // B zones up to this size are read in one pass
 
 __device__ inline uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -442,6 +442,19 @@ __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, ui
     }
 }
 
+// Order of a wave's B zones.  In phase p (0 .. 2nz) the wave of A zone za joins B
+// zone za + dz with (za + dz) = p (mod 2nz + 1): in every phase each B zone that
+// is joined at all is joined by all 2nz + 1 of its A zones at once, and the waves
+// of neighbouring A zones start together, so they read that zone's rows while the
+// first reader's lines are still in the L2.  In zone order (dz = -nz first) the
+// three readers of a B zone met it in three different phases.
+__device__ inline int32_t phase_dz(int32_t p, int32_t za, int32_t nz) {
+    const int32_t m = 2 * nz + 1;
+    int32_t r = (p - za) % m;
+    if (r < 0) r += m;
+    return r <= nz ? r : r - m;
+}
+
 // ---------------------------------------------------------------- dense ----
 // B rows [j0, j1) against the lane's A row (registers), the rows arriving as wide
 // scalar loads from a uniform base, one branch per UNROLL masks.
@@ -565,8 +578,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
             }
             const uint64_t s_lo = uint32_t(s_aq[w][sc]);
             const uint64_t s_hi = uint32_t(s_aq[w][sc + sub - 1]);
-            for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
-                const int32_t zb = za + dz;
+            for (int32_t ph = 0; ph <= 2 * pp.nz; ph++) {
+                const int32_t zb = za + phase_dz(ph, za, pp.nz);
                 if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
                 const uint32_t b0 = uni(bzoff[zb - bzmin]);
                 const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
@@ -587,8 +600,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
 
     const SparseCtx x{start, cnt, half, int32_t(aq), s_aq[w]};
     CandQ q{s_qa[w], s_qb[w], 0, 0};
-    for (int32_t dz = -pp.nz; dz <= pp.nz; dz++) {
-        const int32_t zb = za + dz;
+    for (int32_t ph = 0; ph <= 2 * pp.nz; ph++) {
+        const int32_t zb = za + phase_dz(ph, za, pp.nz);
         if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
         const uint32_t b0 = uni(bzoff[zb - bzmin]);
         const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
