@@ -10,18 +10,21 @@ generated on the device (Philox-4x32-10).
 
 One step = the hot path over one batch, records resident in HBM: prepare catalog
 A (keys, radix sort, zone index, unit vectors), prepare catalog B, join A x B into
-device-resident (idA, idB, dist) triples.  The step runs in both precision forms
-on the same records:
-  fp64_plain                FP64 rows, the FP64 chord on every candidate;
-  fp32_filter_fp64_recheck  the catalogs also carry FP32 rows, written by the same
-                            gather pass as the FP64 rows; the FP32 chord filters
-                            with a guard band, every kept pair is rechecked in FP64.
-Both catalogs are rebuilt in every step of both forms: nothing carries over.
+device-resident (idA, idB, dist) triples.  The step runs in three forms on the
+same records:
+  fp32_filter_fp64_recheck      the FP32 chord filters with a guard band on the FP64
+                                rows rounded to FP32 in registers; every kept pair
+                                is rechecked in FP64 (the library's FP32 form);
+  fp32_filter_stored_fp32_rows  the same filter on FP32 rows that the catalog
+                                builds write in the gather pass beside the FP64
+                                rows (fp32_rows = 1);
+  fp64_plain                    FP64 rows, the FP64 chord on every candidate.
+Both catalogs are rebuilt in every step of every form: nothing carries over.
 
 Metric: pair-tests/s = W_cand / step time.  W_cand = pairs reaching the exact chord
 test under the reference's zone and alpha windows, computed on the host from the
-sorted zone tables.  value = the FP32-filter form's rate; the FP64 form's rate, the
-ratio, the two match sets' digests, the band hits and the largest FP32 chord error
+sorted zone tables.  value = the FP32-filter form's rate; the other forms' rates,
+the ratios, the match sets' digests, the band hits and the largest FP32 chord error
 go beside it, with matches/s and the join-only rates.  Per-kernel durations are
 measured with HIP events on the stream the kernels run on.
 """
@@ -57,13 +60,18 @@ BYTES_PER_SOURCE = {
 }
 KERNELS = ["layout", "make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
            "scan_top", "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
-           "pair_fp64", "pair_fp32", "to_row32", "region_scan", "compact", "job_prep"]
+           "pair_fp64", "pair_fp32", "region_scan", "compact", "job_prep"]
 
 # rocprofv3 names of kernels whose timing name differs
-PROFILE_NAMES = {"pair_fp32": "pair<true>", "pair_fp64": "pair<false>"}
+PROFILE_NAMES = {"pair_fp32": "pair<Fp32Cvt>", "pair_fp64": "pair<Fp64>"}
 
 PLANE_SEED = 2003  # device generator key of the headline catalogs
-MODES = (("fp64_plain", xm.FP64_PLAIN), ("fp32_filter_fp64_recheck", xm.FP32_FILTER_FP64_RECHECK))
+# the forms of the headline join: (name, precision, fp32_rows, --mode key).  The FP32
+# filter runs on the FP64 rows rounded in registers (the library's default) or on
+# FP32 rows stored by the catalog builds (fp32_rows = 1): the same filter values.
+MODES = (("fp32_filter_fp64_recheck", xm.FP32_FILTER_FP64_RECHECK, 0, "fp32"),
+         ("fp32_filter_stored_fp32_rows", xm.FP32_FILTER_FP64_RECHECK, 1, "fp32rows"),
+         ("fp64_plain", xm.FP64_PLAIN, 0, "fp64"))
 
 
 def log(msg):
@@ -217,8 +225,10 @@ def pair_roofline(mode, avg_ms, n_a, n_b, matches, w_tile):
     the rows the form reads) and each match written once as a 24-B (idA, idB,
     chord^2) slot, over the kernel's average launch; and the VALU fraction, W_tile
     tests of FLOP_PER_PAIR_TEST against the vector peak of the form's precision."""
-    if mode == "fp64_plain":
-        per_row, per_match, peak_tf = 8 + 32, 24, FP64_VECTOR_PEAK_TFLOPS
+    if mode in ("fp64_plain", "fp32_filter_fp64_recheck"):
+        # FP64 rows (the FP32 filter rounds them in registers)
+        per_row, per_match = 8 + 32, 24
+        peak_tf = FP64_VECTOR_PEAK_TFLOPS if mode == "fp64_plain" else FP32_VECTOR_PEAK_TFLOPS
         a_bytes, b_bytes = n_a * per_row, n_b * per_row
     else:
         # FP32 rows for the filter, plus the FP64 row of each matched B row for the
@@ -261,8 +271,8 @@ def run_full_sky_device(ctx, n, warmup, steps):
 
 def run_headline(args):
     """The precision form (default): 200M x 200M galactic-plane density at 1",
-    each step building both catalogs and joining, FP64 plain and FP32 filter +
-    FP64 recheck on the same records."""
+    each step building both catalogs and joining, in the FP32 filter + FP64
+    recheck forms and FP64 plain on the same records."""
     n = args.n or 200_000_000
     r = 1.0 / 3600.0
     ctx = xm.Context(0)
@@ -288,9 +298,11 @@ def run_headline(args):
     digests = {}
     dom = None
     # the FP32 form first: its dominant kernel is the one whose launches the timed
-    # steps of both forms carry events around
-    for name, prec in [m for m in MODES[::-1] if args.mode in ("both", m[0].split("_")[0])]:
-        ctx.set_param("fp32_rows", 1 if prec == xm.FP32_FILTER_FP64_RECHECK else 0)
+    # steps of every form carry events around
+    for name, prec, rows32, key in MODES:
+        if args.mode not in ("all", key):
+            continue
+        ctx.set_param("fp32_rows", rows32)
         step = join_step(ctx, dA, dB, r, prec)
         ms, table, only, (dtot, dcnt), (nm, st) = timed_steps(ctx, step, args.warmup,
                                                               args.steps, only=dom)
@@ -315,18 +327,17 @@ def run_headline(args):
         log("%s: %.2f ms per step (join %.2f ms), %d matches"
             % (name, ms, st["join_ms"], nm))
     ctx.set_param("fp32_rows", 0)
-    if args.mode != "both":
+    if args.mode != "all":
         # one form alone (for counter runs that must not mix the forms' kernels)
         print(json.dumps({"mode": args.mode, "modes": modes, "w_cand": w["w_cand"],
                           "w_tile": w["w_tile"], "digests": digests}), flush=True)
         ctx.close()
         return
     f32, f64 = modes["fp32_filter_fp64_recheck"], modes["fp64_plain"]
-    # the dominant kernel of the FP32 form: its algorithmic bytes per launch (the
-    # gather writes the FP32 rows too) over its average launch in the timed steps
-    if dom == "gather":
-        alg = BYTES_PER_SOURCE["gather_fp32"] * n
-    elif dom in BYTES_PER_SOURCE:
+    f32s = modes["fp32_filter_stored_fp32_rows"]
+    # the dominant kernel of the FP32 form: its algorithmic bytes per launch over
+    # its average launch in the timed steps
+    if dom in BYTES_PER_SOURCE:
         alg = BYTES_PER_SOURCE[dom] * n
     elif dom == "pair_fp32":
         alg = f32["pair_roofline"]["algorithmic_bytes_per_launch"]
@@ -356,7 +367,10 @@ def run_headline(args):
         "fp64_plain_value": f64["pair_tests_per_s"],
         "fp32_over_fp64": f32["pair_tests_per_s"] / f64["pair_tests_per_s"],
         "fp32_over_fp64_join_only": f64["join_ms"] / f32["join_ms"],
-        "identical_match_set": digests["fp64_plain"] == digests["fp32_filter_fp64_recheck"],
+        "identical_match_set": (digests["fp64_plain"] == digests["fp32_filter_fp64_recheck"]
+                                == digests["fp32_filter_stored_fp32_rows"]),
+        "fp32_stored_rows_value": f32s["pair_tests_per_s"],
+        "fp32_stored_rows_over_fp64": f32s["pair_tests_per_s"] / f64["pair_tests_per_s"],
         "digest": digests["fp64_plain"],
         "band_hits": f32["band_hits"], "max_dist_err_arcsec": f32["max_dist_err_arcsec"],
         "w_ref": w["w_ref"], "w_cand": w["w_cand"], "w_tile": w["w_tile"],
@@ -491,8 +505,6 @@ def run_batch(args):
         res = ctx.match_batch(A, B, jobs, precision=prec)
         return A, B, res
 
-    # the FP32 form's catalogs carry their FP32 rows from the gather
-    ctx.set_param("fp32_rows", 1 if prec == xm.FP32_FILTER_FP64_RECHECK else 0)
     fixed_ms = None
     if args.autotune:
         # the fixed default shape first, then the tuned shapes (the warmup calls
@@ -776,8 +788,10 @@ def main():
     ap.add_argument("--input-order", choices=["random", "dec"], default="random",
                     help="full_sky config: record order of both catalogs (dec = sorted by "
                          "declination; a diagnostic form)")
-    ap.add_argument("--mode", choices=["both", "fp32", "fp64"], default="both",
-                    help="plane config: both precision forms (the headline line) or one alone")
+    ap.add_argument("--mode", choices=["all", "fp32", "fp32rows", "fp64"], default="all",
+                    help="plane config: every form (the headline line) or one alone: the FP32 "
+                         "filter on FP64 rows rounded in registers, on stored FP32 rows, or "
+                         "FP64 plain")
     ap.add_argument("--no-full-sky", action="store_true",
                     help="plane config: skip the 50M x 50M FP64 figures reported beside it")
     ap.add_argument("--chunk-rows", type=int, default=0,

@@ -38,7 +38,8 @@ typedef enum {
 
 typedef enum {
     XM_FP64_PLAIN = 0,              /* FP64 unit vectors, FP64 chord test on every candidate */
-    XM_FP32_FILTER_FP64_RECHECK = 1 /* FP32 filter with a guard band, FP64 recheck in the band */
+    XM_FP32_FILTER_FP64_RECHECK = 1 /* FP32 chord test with a guard band; every pair it keeps
+                                       is decided and measured again in FP64 */
 } xm_precision;
 
 /* One input record, 24 bytes, packed little-endian.
@@ -113,9 +114,10 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  *                below).  Default 0.
  *   "chunk_rows", "autotune": the pair kernel's shape, see xm_ctx_tune_table.
  *   "fp32_rows"  1 = catalog creation also writes each row's FP32 unit vector (16
- *                bytes a row, in the same pass as the FP64 rows), as the FP32 filter
- *                (XM_FP32_FILTER_FP64_RECHECK) reads them.  0 = they are made on a
- *                catalog's first FP32 join, by a pass of their own.  Default 0.
+ *                bytes a row, in the same pass as the FP64 rows).  The FP32 filter
+ *                (XM_FP32_FILTER_FP64_RECHECK) of two catalogs that carry them reads
+ *                them; otherwise it rounds the FP64 rows to FP32 in registers, the
+ *                same values, and reads no more than the FP64 form.  Default 0.
  *   "sort_fail_test" 1 = every catalog sort of more than one tile (8192 rows)
  *                fails through its look-back's bounded wait, and catalog creation
  *                returns XM_EINTERNAL: for tests of that error path.  Default 0.
@@ -225,8 +227,7 @@ xm_status xm_match_batch(xm_ctx *ctx, const xm_catalog *a, const xm_catalog *b,
  * until the process ends; A's chunk table must fit in it (about 17 MB for 2e8 rows
  * at 1"): a join that needs more returns XM_EINVAL with the sizes.  A join of more
  * pairs than capacity returns XM_ENOMEM with the count in *n_matches and the
- * arrays unwritten.  The FP32 filter needs catalogs built with the parameter
- * fp32_rows = 1.  Every call on a GPU shares that GPU's array: calls that overlap
+ * arrays unwritten.  Every call on a GPU shares that GPU's array: calls that overlap
  * on one GPU (two threads, two contexts of one device) race, and the caller
  * serialises them.  Not timed by xm_ctx_set_timing; stats (may be NULL) carries
  * n_matches, n_chunks, band_hits and max_dist_err_arcsec, join_ms is 0. */

@@ -522,16 +522,18 @@ def test_chunk_rows_rejects_bad_values(gpu_ctx):
             gpu_ctx.set_param("chunk_rows", v)
 
 
-def test_fp32_rows_from_the_gather_equal_the_lazy_ones(gpu_ctx):
-    # catalogs built with the FP32 rows in the gather pass and catalogs whose FP32
-    # rows are made on their first FP32 join give the same pairs and distances
+def test_fp32_rows_and_fp64_rows_rounded_in_registers_agree(gpu_ctx):
+    # the FP32 filter on the catalogs' stored FP32 rows (fp32_rows = 1, written by
+    # the gather) and on the FP64 rows rounded to FP32 in registers: the same
+    # pairs and distances, and -- the FP32 values being the same bits -- the same
+    # band-hit count and largest filter error
     a, b, r = dense_band(32, n=100_000)
     ref, _ = O.match(a, b, r)
-    lazy, _ = gpu_join(gpu_ctx, a, b, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+    cvt, st_cvt = gpu_join(gpu_ctx, a, b, r, precision=xm.FP32_FILTER_FP64_RECHECK)
     gpu_ctx.set_param("fp32_rows", 1)
     try:
         assert gpu_ctx.get_param("fp32_rows") == 1
-        built, st = gpu_join(gpu_ctx, a, b, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+        rows, st_rows = gpu_join(gpu_ctx, a, b, r, precision=xm.FP32_FILTER_FP64_RECHECK)
         seg = synth.make_objs(np.arange(5000), *synth.uniform_patch(
             np.random.default_rng(3), 5000, 100.0, 100.5, 10.0, 10.05))
         cs = gpu_ctx.catalog_segments([a, seg], r)  # the many-segment gather writes them too
@@ -539,12 +541,17 @@ def test_fp32_rows_from_the_gather_equal_the_lazy_ones(gpu_ctx):
         res = gpu_ctx.match_batch(cs, cb, [(0, 0, r)], precision=xm.FP32_FILTER_FP64_RECHECK)
         seg_pairs = res.by_job()[0]
         res.close(); cs.close(); cb.close()
+        ca = gpu_ctx.catalog(a, r)
     finally:
         gpu_ctx.set_param("fp32_rows", 0)
-    assert_same(lazy, ref)
-    assert_same(built, ref)
-    assert_same(seg_pairs, ref)
-    assert st["band_hits"] >= 0
+    cb = gpu_ctx.catalog(b, r)   # without FP32 rows: the join rounds in registers
+    res = gpu_ctx.match(ca, cb, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+    mixed, st_mixed = res.to_host(), res.stats.as_dict()
+    res.close(); ca.close(); cb.close()
+    for got in (cvt, rows, seg_pairs, mixed):
+        assert_same(got, ref)
+    assert st_cvt["band_hits"] == st_rows["band_hits"] == st_mixed["band_hits"]
+    assert st_cvt["max_dist_err_arcsec"] == st_rows["max_dist_err_arcsec"] > 0
 
 
 def test_autotune_keeps_the_sets(gpu_ctx):

@@ -33,11 +33,13 @@ def reference(ctx, ca, cb, r, precision):
     return out
 
 
-@pytest.mark.parametrize("precision", [xm.FP64_PLAIN, xm.FP32_FILTER_FP64_RECHECK],
-                         ids=["fp64", "fp32_filter"])
-def test_noalloc_equals_match_on_two_calls(gpu_ctx, precision):
+@pytest.mark.parametrize("precision,fp32_rows",
+                         [(xm.FP64_PLAIN, 0), (xm.FP32_FILTER_FP64_RECHECK, 0),
+                          (xm.FP32_FILTER_FP64_RECHECK, 1)],
+                         ids=["fp64", "fp32_filter", "fp32_filter_stored_rows"])
+def test_noalloc_equals_match_on_two_calls(gpu_ctx, precision, fp32_rows):
     ctx = gpu_ctx
-    ctx.set_param("fp32_rows", 1)
+    ctx.set_param("fp32_rows", fp32_rows)
     try:
         big = synth.config_small_patch(seed=61, n_a=300_000, n_b=300_000)
         small = synth.config_small_patch(seed=62, n_a=40_000, n_b=50_000)
@@ -74,10 +76,11 @@ def test_noalloc_rejects_what_it_cannot_hold(gpu_ctx):
         ctx.match_noalloc(ca, cb, r, *small, 100)
     assert e.value.status == xm.XM_ENOMEM and e.value.n_matches == len(ref[0])
     assert np.all(ctx.download(small[0]) == SENTINEL)   # nothing written
-    # FP32 rows were not built with these catalogs: rejected, not made
+    # the FP32 form (FP64 rows rounded in registers) counts the same pairs
     with pytest.raises(xm.XmatchError) as e:
         ctx.match_noalloc(ca, cb, r, *small, 100, precision=xm.FP32_FILTER_FP64_RECHECK)
-    assert e.value.status == xm.XM_EINVAL
+    assert e.value.status == xm.XM_ENOMEM and e.value.n_matches == len(ref[0])
+    assert np.all(ctx.download(small[0]) == SENTINEL)
     # a join after the rejections still equals xm_match
     bufs = out_buffers(ctx, 50_000)
     n, _ = ctx.match_noalloc(ca, cb, r, *bufs, 50_000)

@@ -1098,8 +1098,8 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
         xm_catalog* Bm = const_cast<xm_catalog*>(b);
         // A catalog of another context on this device is read and never changed:
         // what the join caches in a catalog of its own context -- A's chunk table
-        // (with its segment table's chunk rows) and FP32 rows -- is made in this
-        // context's memory for the join's duration.
+        // (with its segment table's chunk rows) -- is made in this context's memory
+        // for the join's duration.
         if (a->ctx != ctx) {
             a_view = *a;
             a_view.ctx = ctx;
@@ -1115,17 +1115,6 @@ xm_status match_jobs(xm_ctx* ctx, const xm_catalog* a, const xm_catalog* b, cons
             b_view = *b;
             b_view.ctx = ctx;
             Bm = &b_view;
-        }
-        if (f32) {
-            for (xm_catalog* c : {A, Bm}) {
-                if (c->row32 || c->n == 0) continue;
-                const bool view = c == &a_view || c == &b_view;
-                if (view) local.reserve(local.size() + 1);
-                c->row32 = dalloc(ctx, c->n * sizeof(Row32));
-                if (view) local.push_back(c->row32);
-                launch_to_row32(ctx, static_cast<const Row64*>(c->row), c->n,
-                                static_cast<Row32*>(c->row32));
-            }
         }
         const xm_catalog* B = Bm;
         if (ctx->autotune) tune_jobs(ctx, A, B, jobs, n_jobs, f32);

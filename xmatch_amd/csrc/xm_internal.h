@@ -117,8 +117,9 @@ struct xm_catalog {
     uint64_t* key = nullptr;         // sorted catalog keys (seg << 48 | zone << 24 | ra)
     void* row = nullptr;             // xm::Row64[n]: FP64 unit vector + id, sorted order
     void* row32 = nullptr;           // xm::Row32[n]: FP32 unit vectors, written by the gather
-                                     // when the context's fp32_rows is set, else made on the
-                                     // first FP32 join
+                                     // when the context's fp32_rows is set (FP32 joins of two
+                                     // such catalogs read them; otherwise they round the FP64
+                                     // rows in registers)
     uint32_t* zoff = nullptr;        // [n_zoff]: first row of each segment zone, then n
     uint64_t n_zoff = 0;             // sum over segments of (zmax - zmin + 1), plus 1
     // A-side row chunks (built on first use as the A catalog of a join): the
@@ -459,12 +460,12 @@ void launch_job_prep(xm_ctx* ctx, JobDesc* jobs, uint32_t n_jobs, const uint32_t
 uint32_t pair_blocks(uint64_t n_waves);
 // One launch over every (job, A chunk): wave g belongs to the job j with
 // job_prefix[j] <= g < job_prefix[j+1] (job_prefix[n_jobs] = total, on the
-// device); the grid covers max_waves.  f32 = FP32 filter + FP64 recheck (needs
-// row32 of both catalogs), else FP64 plain.
+// device); the grid covers max_waves.  f32 = FP32 filter + FP64 recheck, on the
+// FP32 rows when both catalogs carry them, else on the FP64 rows rounded to FP32 in
+// registers (the same values); otherwise FP64 plain.
 void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Chunk* chunks,
                  const JobDesc* jobs, const uint32_t* job_prefix, uint32_t n_jobs,
                  uint64_t max_waves, bool f32, const RegionOut& ro);
-void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out);
 // d[i] = 2 asin(sqrt(d[i]) / 2) in arcsec for i < n (chord^2 to distance, in place).
 void launch_chord_to_arcsec(xm_ctx* ctx, double* d, uint64_t n);
 // Regions of cap_each slots back to back, then SPILL_POOLS pools of pool_cap.

@@ -126,9 +126,6 @@ extern "C" xm_status xm_match_noalloc(xm_ctx* ctx, const xm_catalog* a, const xm
         if (precision != XM_FP64_PLAIN && precision != XM_FP32_FILTER_FP64_RECHECK)
             throw InvalidArg{"unknown precision mode"};
         const bool f32 = precision == XM_FP32_FILTER_FP64_RECHECK;
-        if (f32 && ((a->n && !a->row32) || (b->n && !b->row32)))
-            throw InvalidArg{"the FP32 filter without allocation needs catalogs built with the "
-                             "context parameter fp32_rows = 1"};
         if (capacity && (!d_id_a || !d_id_b || !d_dist_arcsec))
             throw InvalidArg{"null output arrays"};
         if (!ctx->noalloc_scratch) throw InvalidArg{"the context has no allocation-free scratch"};

@@ -111,14 +111,31 @@ __device__ inline T chord2(T ax, T ay, T az, T bx, T by, T bz) {
     return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
 
-template <bool F32> struct Prec;
-template <> struct Prec<false> {
+// The three forms of a join: the test's arithmetic (T), the rows the test reads
+// (Row) and whether it is the FP32 filter (kept pairs rechecked in FP64 when
+// written).  Fp32Cvt rounds the FP64 rows to FP32 in registers, exactly as the
+// FP32 rows are made, so its filter decisions are those of Fp32Rows bit for bit;
+// it reads the FP64 rows the recheck needs anyway, where Fp32Rows reads 16-byte
+// rows for the test and the FP64 rows of every kept pair on top.
+// In Fp32Cvt's sparse path both FP64 rows are in registers at the test, so the
+// recheck runs there (REG) and only final pairs are staged.
+struct Fp64 {
     using T = double;
     using Row = Row64;
+    static constexpr bool F32 = false;
+    static constexpr bool REG = false;
 };
-template <> struct Prec<true> {
+struct Fp32Rows {
     using T = float;
     using Row = Row32;
+    static constexpr bool F32 = true;
+    static constexpr bool REG = false;
+};
+struct Fp32Cvt {
+    using T = float;
+    using Row = Row64;
+    static constexpr bool F32 = true;
+    static constexpr bool REG = true;
 };
 
 struct Stage {
@@ -374,14 +391,70 @@ struct CandQ {
     uint32_t head, tail;      // wave-uniform
 };
 
-template <bool F32>
+// The FP32 filter's statistics of the pairs a lane rechecked in registers: pairs
+// kept with the FP32 chord at or above the band's lower edge, and the largest
+// |chord_32 - chord_64| of a match -- as flush_stage counts them for staged pairs.
+struct FilterStats {
+    uint32_t band = 0;
+    double err = 0.0;
+};
+
+template <typename P>
+__device__ inline void flush_filter_stats(const FilterStats& fs, uint32_t lane, const Out& o) {
+    if constexpr (P::REG) {
+        uint32_t band = fs.band;
+        double err = fs.err;
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) {
+            band += __shfl_xor(band, d, WAVE);
+            const double e2 = __shfl_xor(err, d, WAVE);
+            err = e2 > err ? e2 : err;
+        }
+        if (lane == 0) {
+            if (band) atomicAdd(o.counter + 1, (unsigned long long)band);
+            if (err > 0.0)
+                atomicMax(o.counter + 2, (unsigned long long)__double_as_longlong(err));
+        }
+    }
+}
+
+template <typename P>
 __device__ inline void cand_round(CandQ& q, uint32_t k,
-                                  const typename Prec<F32>::Row* __restrict__ arow,
-                                  const typename Prec<F32>::Row* __restrict__ brow,
-                                  typename Prec<F32>::T thr, uint32_t lane, Stage& st,
-                                  const Out& o) {
-    using T = typename Prec<F32>::T;
+                                  const typename P::Row* __restrict__ arow,
+                                  const typename P::Row* __restrict__ brow,
+                                  typename P::T thr, uint32_t lane, Stage& st,
+                                  const Out& o, FilterStats& fs) {
+    using T = typename P::T;
     wave_lds_sync();  // the ring entries written by other lanes
+    if constexpr (P::REG) {
+        // the FP32 filter on the rows rounded to FP32; the pairs it keeps are
+        // decided by the FP64 chord of the same rows, and only matches are staged
+        double d64 = 0.0;
+        bool hit = false;
+        uint32_t ra = 0, rb = 0;
+        if (lane < k) {
+            const uint32_t s = (q.head + lane) & (CQ - 1);
+            ra = q.a[s];
+            rb = q.b[s];
+            const Row64 a = arow[ra];
+            const Row64 b = brow[rb];
+            const float d32 = chord2<float>(a.x, a.y, a.z, b.x, b.y, b.z);
+            if (d32 < thr) {
+                const double dd = double(d32);
+                fs.band += dd >= o.t_lo ? 1u : 0u;
+                d64 = chord2<double>(a.x, a.y, a.z, b.x, b.y, b.z);
+                hit = d64 < o.thr;
+                if (hit) {
+                    const double e1 = __builtin_fabs(__builtin_sqrt(dd) - __builtin_sqrt(d64));
+                    fs.err = e1 > fs.err ? e1 : fs.err;
+                }
+            }
+        }
+        q.head += k;
+        const uint64_t m = __ballot(hit);
+        if (m) stage_hits<false>(m, d64, ra, rb, lane, st, o);
+        return;
+    }
     T d2 = 0;
     bool hit = false;
     uint32_t ra = 0, rb = 0;
@@ -396,21 +469,21 @@ __device__ inline void cand_round(CandQ& q, uint32_t k,
     }
     q.head += k;
     const uint64_t m = __ballot(hit);
-    if (m) stage_hits<F32>(m, double(d2), ra, rb, lane, st, o);
+    if (m) stage_hits<P::F32>(m, double(d2), ra, rb, lane, st, o);
 }
 
 // One B row per lane (row j, key), against the chunk rows within +-aq of its ra if
 // it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).  The chunk's
 // ra quanta are in LDS: a binary search finds the first, a scan the rest, and each
 // candidate (A row, B row) goes to the ring; full rounds are tested at once.
-template <bool F32>
+template <typename P>
 __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key,
                                   int ni, const int32_t* lq, const int32_t* hq,
                                   const int32_t* shift, CandQ& q,
-                                  const typename Prec<F32>::Row* __restrict__ arow,
-                                  const typename Prec<F32>::Row* __restrict__ brow,
-                                  typename Prec<F32>::T thr, uint32_t lane, Stage& st,
-                                  const Out& o) {
+                                  const typename P::Row* __restrict__ arow,
+                                  const typename P::Row* __restrict__ brow,
+                                  typename P::T thr, uint32_t lane, Stage& st,
+                                  const Out& o, FilterStats& fs) {
     const int32_t qv = int32_t(key & RA_MASK);
     bool in = false;
     int32_t qb = 0;
@@ -438,7 +511,8 @@ __device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, ui
             q.b[s] = j;
         }
         q.tail += uint32_t(__popcll(m));
-        if (q.tail - q.head >= uint32_t(WAVE)) cand_round<F32>(q, WAVE, arow, brow, thr, lane, st, o);
+        if (q.tail - q.head >= uint32_t(WAVE))
+            cand_round<P>(q, WAVE, arow, brow, thr, lane, st, o, fs);
     }
 }
 
@@ -458,14 +532,14 @@ __device__ inline int32_t phase_dz(int32_t p, int32_t za, int32_t nz) {
 // ---------------------------------------------------------------- dense ----
 // B rows [j0, j1) against the lane's A row (registers), the rows arriving as wide
 // scalar loads from a uniform base, one branch per UNROLL masks.
-template <bool F32>
-__device__ inline void dense_range(uint32_t j0, uint32_t j1, typename Prec<F32>::T ax,
-                                   typename Prec<F32>::T ay, typename Prec<F32>::T az,
-                                   uint32_t row, typename Prec<F32>::T thr,
-                                   const typename Prec<F32>::Row* __restrict__ brow,
+template <typename P>
+__device__ inline void dense_range(uint32_t j0, uint32_t j1, typename P::T ax,
+                                   typename P::T ay, typename P::T az,
+                                   uint32_t row, typename P::T thr,
+                                   const typename P::Row* __restrict__ brow,
                                    uint32_t lane, Stage& st, const Out& o) {
-    using T = typename Prec<F32>::T;
-    using Row = typename Prec<F32>::Row;
+    using T = typename P::T;
+    using Row = typename P::Row;
     uint32_t j = j0;
     for (; j + UNROLL <= j1; j += UNROLL) {
         const Row* __restrict__ pb = brow + size_t(j);
@@ -481,31 +555,31 @@ __device__ inline void dense_range(uint32_t j0, uint32_t j1, typename Prec<F32>:
         if (any) {
 #pragma unroll
             for (int u = 0; u < UNROLL; u++)
-                if (m[u]) stage_hits<F32>(m[u], double(d2[u]), row, j + u, lane, st, o);
+                if (m[u]) stage_hits<P::F32>(m[u], double(d2[u]), row, j + u, lane, st, o);
         }
     }
     for (; j < j1; j++) {
         const T d2 = chord2<T>(ax, ay, az, brow[j].x, brow[j].y, brow[j].z);
         const uint64_t m = __ballot(d2 < thr);
-        if (m) stage_hits<F32>(m, double(d2), row, j, lane, st, o);
+        if (m) stage_hits<P::F32>(m, double(d2), row, j, lane, st, o);
     }
 }
 
 // ----------------------------------------------------------------- join ----
 // One wavefront per (job, A chunk).  The wave total lives on the device
 // (job_prefix[n_jobs]); the grid covers an upper bound and the waves past the
-// total return at once.  F32 selects the FP32 filter: tests read the catalogs'
-// FP32 rows (arow_t/brow_t) and keep pairs below t_hi32; the FP64 rows serve ids
-// and the recheck.
-template <bool F32>
+// total return at once.  P is the form: the tests read arow_t/brow_t (the FP64
+// rows, or for Fp32Rows the catalogs' FP32 rows) and, in the FP32 forms, keep
+// pairs below t_hi32; the FP64 rows serve ids and the recheck.
+template <typename P>
 __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const JobDesc* __restrict__ jobs, const uint32_t* __restrict__ job_prefix, uint32_t n_jobs,
     const Chunk* __restrict__ chunks, const uint64_t* __restrict__ akey,
-    const Row64* __restrict__ arow, const typename Prec<F32>::Row* __restrict__ arow_t,
+    const Row64* __restrict__ arow, const typename P::Row* __restrict__ arow_t,
     const uint64_t* __restrict__ bkey, const Row64* __restrict__ brow,
-    const typename Prec<F32>::Row* __restrict__ brow_t, const uint32_t* __restrict__ zoff_b,
+    const typename P::Row* __restrict__ brow_t, const uint32_t* __restrict__ zoff_b,
     RegionOut ro) {
-    using T = typename Prec<F32>::T;
+    using T = typename P::T;
     __shared__ uint32_t s_ra[PK_WAVES][STAGE];
     __shared__ uint32_t s_rb[PK_WAVES][STAGE];
     __shared__ double s_d2[PK_WAVES][STAGE];
@@ -559,7 +633,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const uint64_t q_hi = uint32_t(s_aq[w][cnt - 1]);
     const int64_t aq = join_quanta(pp.theta_deg, pp.h_deg, za, pp.atan_sin_theta, pp.gran);
     const bool dense = chunk_is_dense(q_lo, q_hi, aq, pp.dense_min, cnt);
-    const T thr = F32 ? T(pp.t_hi32) : T(pp.chord2);
+    const T thr = P::F32 ? T(pp.t_hi32) : T(pp.chord2);
     Stage st{s_ra[w], s_rb[w], s_d2[w], 0};
     const Out o = make_out(arow, brow, ro, pp, jid, g);
 
@@ -590,16 +664,17 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
                 for (int t = 0; t < ni; t++) {
                     const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t] | bpre, lane);
                     const uint32_t j1 = wave_lower_bound(bkey, j0, b1, (hi[t] | bpre) + 1u, lane);
-                    dense_range<F32>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
+                    dense_range<P>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
                 }
             }
         }
-        if (st.n) flush_stage<F32>(st, lane, o);
+        if (st.n) flush_stage<P::F32>(st, lane, o);
         return;
     }
 
     const SparseCtx x{start, cnt, half, int32_t(aq), s_aq[w]};
     CandQ q{s_qa[w], s_qb[w], 0, 0};
+    FilterStats fs;
     for (int32_t ph = 0; ph <= 2 * pp.nz; ph++) {
         const int32_t zb = za + phase_dz(ph, za, pp.nz);
         if (zb < 0 || zb >= pp.n_zones || zb < bzmin || zb > bzmax) continue;
@@ -627,8 +702,8 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 if (b0 + k * WAVE >= b1) break;
                 const uint32_t j = b0 + k * WAVE + lane;
-                sparse_row<F32>(x, j, j < b1, kk[k], ni, lq, hq, shift, q, arow_t, brow_t, thr,
-                                lane, st, o);
+                sparse_row<P>(x, j, j < b1, kk[k], ni, lq, hq, shift, q, arow_t, brow_t, thr,
+                              lane, st, o, fs);
             }
         } else {
             for (int t = 0; t < ni; t++) {
@@ -638,17 +713,19 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
                     const uint32_t j = jb + lane;
                     const bool valid = j < j1;
                     const uint64_t key = valid ? bkey[j] : 0;
-                    sparse_row<F32>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], q, arow_t,
-                                    brow_t, thr, lane, st, o);
+                    sparse_row<P>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], q, arow_t,
+                                  brow_t, thr, lane, st, o, fs);
                 }
             }
         }
     }
     while (q.tail != q.head) {
         const uint32_t k = q.tail - q.head < uint32_t(WAVE) ? q.tail - q.head : uint32_t(WAVE);
-        cand_round<F32>(q, k, arow_t, brow_t, thr, lane, st, o);
+        cand_round<P>(q, k, arow_t, brow_t, thr, lane, st, o, fs);
     }
-    if (st.n) flush_stage<F32>(st, lane, o);
+    // staged pairs are final where the recheck ran in registers
+    if (st.n) flush_stage<P::F32 && !P::REG>(st, lane, o);
+    flush_filter_stats<P>(fs, lane, o);
 }
 
 // One block.  Parts: the n_regions regions (kept = min(count, cap)), then the
@@ -763,20 +840,6 @@ __global__ __launch_bounds__(256) void k_chord_to_arcsec(double* __restrict__ d,
     if (i < n) d[i] = dist_arcsec(d[i]);
 }
 
-// FP64 row -> FP32 row (round to nearest), for the FP32 filter.
-__global__ __launch_bounds__(256) void k_to_row32(const Row64* __restrict__ in, uint64_t n,
-                                                  Row32* __restrict__ out) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n) return;
-    const Row64 r = in[i];
-    Row32 o;
-    o.x = float(r.x);
-    o.y = float(r.y);
-    o.z = float(r.z);
-    o.pad = 0u;
-    out[i] = o;
-}
-
 }  // namespace
 
 void launch_compact(xm_ctx* ctx, const RegionOut& ro, const uint64_t* prefix, uint64_t total,
@@ -843,13 +906,6 @@ void launch_chord_to_arcsec(xm_ctx* ctx, double* d, uint64_t n) {
     });
 }
 
-void launch_to_row32(xm_ctx* ctx, const Row64* in, uint64_t n, Row32* out) {
-    if (n == 0) return;
-    launch_timed(ctx, "to_row32", [&] {
-        hipLaunchKernelGGL(k_to_row32, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                           in, n, out);
-    });
-}
 
 void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Chunk* chunks,
                  const JobDesc* jobs, const uint32_t* job_prefix, uint32_t n_jobs,
@@ -858,16 +914,22 @@ void launch_pair(xm_ctx* ctx, const xm_catalog* A, const xm_catalog* B, const Ch
     const dim3 grid(pair_blocks(max_waves));
     const Row64* arow = static_cast<const Row64*>(A->row);
     const Row64* brow = static_cast<const Row64*>(B->row);
-    if (f32) {
+    if (f32 && A->row32 && B->row32) {
         launch_timed(ctx, "pair_fp32", [&] {
-            hipLaunchKernelGGL(k_pair<true>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
+            hipLaunchKernelGGL(k_pair<Fp32Rows>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
                                job_prefix, n_jobs, chunks, A->key, arow,
                                static_cast<const Row32*>(A->row32), B->key, brow,
                                static_cast<const Row32*>(B->row32), B->zoff, ro);
         });
+    } else if (f32) {
+        launch_timed(ctx, "pair_fp32", [&] {
+            hipLaunchKernelGGL(k_pair<Fp32Cvt>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
+                               job_prefix, n_jobs, chunks, A->key, arow, arow, B->key, brow, brow,
+                               B->zoff, ro);
+        });
     } else {
         launch_timed(ctx, "pair_fp64", [&] {
-            hipLaunchKernelGGL(k_pair<false>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
+            hipLaunchKernelGGL(k_pair<Fp64>, grid, dim3(PK_THREADS), 0, ctx->stream, jobs,
                                job_prefix, n_jobs, chunks, A->key, arow, arow, B->key, brow, brow,
                                B->zoff, ro);
         });

@@ -366,8 +366,10 @@ xm_status xm_server_create(xm_ctx* ctx, const xm_catalog* resident, const xm_ser
         delete s;
         return st;
     }
-    // FP32 form: the query catalogs carry their FP32 rows from the gather
-    if (s->cfg.precision == XM_FP32_FILTER_FP64_RECHECK)
+    // FP32 form over a resident catalog stored with FP32 rows: the query catalogs
+    // get theirs in the gather, so the filter reads FP32 rows on both sides (else it
+    // rounds the FP64 rows in registers)
+    if (s->cfg.precision == XM_FP32_FILTER_FP64_RECHECK && resident->row32)
         (void)xm_ctx_set_param(s->prep, "fp32_rows", 1);
     try {
         s->preparer = std::thread([s] { s->prepare_loop(); });
