@@ -69,7 +69,11 @@ def test_sorted_by_zone_then_ra(gpu_ctx):
     ra = dict(zip(A["id"].tolist(), A["ra"].tolist()))
     ras = np.array([ra[i] for i in e["id"]])
     same = np.diff(e["zone"]) == 0
-    assert np.all(np.diff(ras)[same] >= -360.0 / 2**24)
+    # ra order within a zone, to the sort key's granularity: this catalog's key
+    # has 24 ra bits, of which the layout drops fewer than one digit (at most 8),
+    # rows of one quantum keeping input order
+    assert np.all(np.diff(ras)[same] >= -360.0 / 2**16)
+    assert np.mean(np.diff(ras)[same] >= 0) > 0.99
 
 
 def test_sort_is_stable_and_word_width_invariant(gpu_ctx):

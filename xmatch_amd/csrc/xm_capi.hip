@@ -361,16 +361,19 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     // the sort's digit counts, taken while the keys are written: digit positions
     // from the row bits up over the ra bits and every zone bit (the sort may need
     // fewer once the occupied zone range is known)
-    const int hist_pass = (pl.ra_bits + pl.zone_bits + 7) / 8;
-    // scratch: zone range, bad count and dropped ra bits; layout statistics; digit counts
+    // (8-bit digits take the most passes; 9-bit ones the most counters per pass)
+    const int hist_pass = sort_passes(pl.ra_bits + pl.zone_bits, 8);
+    const size_t digit_words = size_t(hist_pass) * 512;
+    // scratch: zone range, bad count and the layout word; layout statistics; digit counts
     uint32_t* scratch = tmp.alloc<uint32_t>(
-        (BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + size_t(hist_pass) * 256) * sizeof(uint32_t));
-    launch_build_init(ctx, scratch, uint64_t(hist_pass) * 256);
+        (BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS + digit_words) * sizeof(uint32_t));
+    launch_build_init(ctx, scratch, digit_words);
     int32_t* small = reinterpret_cast<int32_t*>(scratch);
     uint32_t* stats = scratch + BUILD_SMALL_WORDS;
     uint32_t* digit_totals = stats + LAYOUT_STATS_WORDS;
-    // the ra bits the sort and the key drop, decided on the device from a sample
-    launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, stats,
+    // the ra bits the sort and the key drop and the sort's digit width, decided on
+    // the device from a sample
+    launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, pl.zone_bits, stats,
                   reinterpret_cast<uint32_t*>(small + 3));
     launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
                      reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
@@ -380,7 +383,8 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     sync(ctx);
     const int32_t zmin = pin[0], zmax = pin[1];
     const uint32_t nbad = uint32_t(pin[2]);
-    const int ra_drop = pin[3];
+    const int ra_drop = layout_drop(uint32_t(pin[3]));
+    const int digit_bits = layout_digit_bits(uint32_t(pin[3]));
     d.ra_bits = pl.ra_bits - ra_drop;
     d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     if (nbad) {
@@ -395,7 +399,8 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     // order ties, which keep input order).
     const int zone_lo = pl.ra_bits + pl.idx_bits;
     const int hi_bit = zone_lo + bit_length(uint32_t(zmin) ^ uint32_t(zmax));
-    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals);
+    radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals,
+                    digit_bits);
     tmp.free(scratch);
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));

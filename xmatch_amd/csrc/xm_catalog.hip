@@ -31,6 +31,10 @@ constexpr int TPB = 256;
 // Measured on 50M records: 1024 blocks 0.300 ms, 768 0.318, 512 0.391, 2048 0.320,
 // 4096 0.328.
 constexpr uint32_t MAKE_KEYS_BLOCKS = 1024;
+// digit counters per histogram copy: every pass of 8-bit digits, or up to
+// SORT9_MAX_PASSES of 9-bit digits
+constexpr int MAKE_KEYS_HIST_BINS =
+    SORT_MAX_PASSES * 256 > SORT9_MAX_PASSES * 512 ? SORT_MAX_PASSES * 256 : SORT9_MAX_PASSES * 512;
 
 __device__ inline bool valid_record(double ra, double dec) {
     return ra >= 0.0 && ra < 360.0 && dec >= -90.0 && dec <= 90.0;  // false for NaN
@@ -38,10 +42,12 @@ __device__ inline bool valid_record(double ra, double dec) {
 
 // Grid-stride over the records with a capped grid, so the zone range and the
 // bad-record count cost one atomic per block rather than one per 256 records.
-// ---- one segment: how many low ra bits the sort key drops
-// The sort runs over the zone and ra bits in 8-bit digits.  When they are a few
-// bits over a whole number of digits, dropping those ra bits saves a pass; the
-// coarser ra widens each B row's search by the granularity.  k_layout_sample
+// ---- one segment: how many low ra bits the sort key drops, and the digit width
+// The sort runs over the zone and ra bits in 8- or 9-bit digits.  When they are a
+// few bits over a whole number of digits, dropping those ra bits saves a pass; the
+// coarser ra widens each B row's search by the granularity.  9-bit digits are
+// taken where they need fewer passes than 8-bit ones (the 200M-row plane catalog
+// at 1": 36 bits, 4 passes instead of 5).  k_layout_sample
 // takes LAYOUT_SAMPLE records at an even stride: histograms of their zone (256
 // bins over all zones) and ra (256 bins over the circle), their zone range and
 // count.  k_layout_decide (one thread) estimates the rows a search meets per
@@ -96,8 +102,8 @@ __global__ __launch_bounds__(TPB) void k_layout_sample(const xm_obj* __restrict_
 }
 
 __global__ __launch_bounds__(TPB) void k_layout_decide(uint32_t* __restrict__ stats, uint64_t n,
-                                                       int32_t n_zones, int ra_bits,
-                                                       uint32_t* __restrict__ drop_out) {
+                                                       int32_t n_zones, int ra_bits, int zone_bits,
+                                                       uint32_t* __restrict__ layout_out) {
     __shared__ double s_z[TPB / WAVE], s_r[TPB / WAVE];
     const double hz = double(stats[threadIdx.x]), hr = double(stats[256 + threadIdx.x]);
     double sz = hz * hz, sr = hr * hr;
@@ -113,22 +119,41 @@ __global__ __launch_bounds__(TPB) void k_layout_decide(uint32_t* __restrict__ st
     __syncthreads();
     if (threadIdx.x != 0) return;
     const uint32_t m = stats[514];
-    uint32_t drop = 0;
+    uint32_t drop = 0, digit_bits = 8;
     if (m > 0) {
         sz = s_z[0] + s_z[1] + s_z[2] + s_z[3];
         sr = s_r[0] + s_r[1] + s_r[2] + s_r[3];
         const double mm = double(m) * double(m);
         const double conc = (256.0 * sz / mm) * (256.0 * sr / mm);
         const int32_t zmn = int32_t(stats[512]), zmx = int32_t(stats[513]);
-        const int d = (ra_bits + bit_length_u64(uint64_t(uint32_t(zmn) ^ uint32_t(zmx)))) % 8;
-        if (d > 0 && d < ra_bits) {
-            const double granule = ldexp(1.0, d - ra_bits);  // of the circle
-            const double extra = double(n) / double(n_zones) * conc * 2.0 * granule;
-            if (extra <= LAYOUT_MAX_EXTRA) drop = uint32_t(d);
+        const int bits = ra_bits + bit_length_u64(uint64_t(uint32_t(zmn) ^ uint32_t(zmx)));
+        // for each digit width, the ra bits to drop to reach a whole number of
+        // digits (if the rows each search meets grow little) and the passes left;
+        // 9-bit digits only where they save a pass
+        int best = SORT_MAX_PASSES + 1;
+        for (int db = 8; db <= 9; db++) {
+            int d = bits % db;
+            if (d > 0) {
+                bool ok = d < ra_bits;
+                if (ok) {
+                    const double granule = ldexp(1.0, d - ra_bits);  // of the circle
+                    const double extra = double(n) / double(n_zones) * conc * 2.0 * granule;
+                    ok = extra <= LAYOUT_MAX_EXTRA;
+                }
+                if (!ok) d = 0;
+            }
+            // the make_keys histograms cover the whole zone field
+            if (db == 9 && sort_passes(ra_bits - d + zone_bits, 9) > SORT9_MAX_PASSES) continue;
+            const int passes = sort_passes(bits - d, db);
+            if (passes < best) {
+                best = passes;
+                drop = uint32_t(d);
+                digit_bits = uint32_t(db);
+            }
         }
     }
     stats[515] = drop;
-    *drop_out = drop;
+    *layout_out = drop | digit_bits << 8;
 }
 
 // The block also counts the digits of its valid words for the radix sort: four
@@ -143,19 +168,25 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
                                                    int32_t* __restrict__ zminmax,
                                                    uint32_t* __restrict__ bad, int lo_bit,
                                                    int n_pass, uint32_t* __restrict__ digit_totals,
-                                                   const uint32_t* __restrict__ ra_drop) {
-    // the sort starts above the ra bits the key drops (their count is decided on
-    // the device); the digit counts follow
-    if (ra_drop) {
-        const int drop = int(*ra_drop);
+                                                   const uint32_t* __restrict__ layout) {
+    // the sort starts above the ra bits the key drops, in digits of the width the
+    // layout chose (both decided on the device); the digit counts follow
+    int db = 8;
+    if (layout) {
+        const uint32_t lw = *layout;
+        const int drop = layout_drop(lw);
+        db = layout_digit_bits(lw);
         lo_bit += drop;
-        const int need = (pl.ra_bits - drop + pl.zone_bits + 7) / 8;
+        const int need = sort_passes(pl.ra_bits - drop + pl.zone_bits, db);
         n_pass = n_pass < need ? n_pass : need;
     }
+    const uint32_t radix = 1u << db;
+    const uint32_t dmask = radix - 1u;
     __shared__ int32_t s_min[TPB / WAVE], s_max[TPB / WAVE];
     __shared__ uint32_t s_bad[TPB / WAVE];
-    __shared__ uint32_t s_h[4][SORT_MAX_PASSES * 256];
-    for (int i = threadIdx.x; i < n_pass * 256; i += TPB)
+    __shared__ uint32_t s_h[4][MAKE_KEYS_HIST_BINS];
+    const int bins = n_pass * int(radix);
+    for (int i = threadIdx.x; i < bins; i += TPB)
 #pragma unroll
         for (int c = 0; c < 4; c++) s_h[c][i] = 0;
     __syncthreads();
@@ -173,7 +204,8 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
             zmn = min(zmn, z);
             zmx = max(zmx, z);
             for (int p = 0; p < n_pass; p++)
-                atomicAdd(&h[p * 256 + (uint32_t(word >> (lo_bit + 8 * p)) & 255u)], 1u);
+                atomicAdd(&h[uint32_t(p) * radix + (uint32_t(word >> (lo_bit + db * p)) & dmask)],
+                          1u);
         } else {
             nbad++;
         }
@@ -206,7 +238,7 @@ __global__ __launch_bounds__(TPB) void k_make_keys(const xm_obj* __restrict__ re
         }
         if (nb) atomicAdd(bad, nb);
     }
-    for (int i = threadIdx.x; i < n_pass * 256; i += TPB)
+    for (int i = threadIdx.x; i < bins; i += TPB)
         if (const uint32_t t = s_h[0][i] + s_h[1][i] + s_h[2][i] + s_h[3][i])
             atomicAdd(&digit_totals[i], t);
 }
@@ -321,7 +353,7 @@ __global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ s
                 const uint64_t q = ra_quantum(ra) >> (RA_BITS - rb);
                 const uint64_t word = top | (uint64_t(uint32_t(z)) << (rb + ib)) | (q << ib) | (i - base);
                 words[i] = word;
-                count_digits(s_h, word, true, lo_bit, n_pass);
+                count_digits<8>(s_h, word, true, lo_bit, n_pass);
             }
         }
     }
@@ -502,8 +534,8 @@ __global__ __launch_bounds__(TPB) void k_fill(T* __restrict__ p, uint64_t n, T v
 inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 
 // The one-segment build's scratch words, set in one launch rather than one fill
-// each: small = {INT_MAX, -1, 0, 0} (zone min, zone max, bad records, ra bits
-// dropped), then the layout statistics (zero but zone min INT_MAX and zone max -1),
+// each: small = {INT_MAX, -1, 0, 0} (zone min, zone max, bad records, layout),
+// then the layout statistics (zero but zone min INT_MAX and zone max -1),
 // then the digit counters (zero).
 __global__ __launch_bounds__(TPB) void k_build_init(uint32_t* __restrict__ w, uint32_t n_words) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
@@ -519,7 +551,7 @@ __global__ __launch_bounds__(TPB) void k_build_init(uint32_t* __restrict__ w, ui
 
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad, int lo_bit,
-                      int n_pass, uint32_t* digit_totals, const uint32_t* ra_drop) {
+                      int n_pass, uint32_t* digit_totals, const uint32_t* layout) {
     if (n == 0) return;
     if (n_pass < 0 || n_pass > SORT_MAX_PASSES || (n_pass > 0 && !digit_totals))
         throw InvalidArg{"make_keys: bad digit histogram request"};
@@ -527,7 +559,7 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
     launch_timed(ctx, "make_keys", [&] {
         hipLaunchKernelGGL(k_make_keys, dim3(nblk), dim3(TPB), 0, ctx->stream, recs, uint32_t(n),
                            h_deg, n_zones, pl, words, zminmax, bad, lo_bit, n_pass, digit_totals,
-                           ra_drop);
+                           layout);
     });
 }
 
@@ -540,14 +572,14 @@ void launch_build_init(xm_ctx* ctx, uint32_t* scratch, uint64_t n_digit_words) {
 }
 
 void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
-                   int ra_bits, uint32_t* stats, uint32_t* drop_out) {
+                   int ra_bits, int zone_bits, uint32_t* stats, uint32_t* layout_out) {
     if (n == 0) return;
     const uint32_t m = uint32_t(std::min<uint64_t>(n, LAYOUT_SAMPLE));
     launch_timed(ctx, "layout", [&] {
         hipLaunchKernelGGL(k_layout_sample, dim3(64), dim3(TPB), 0, ctx->stream, recs, n, m,
                            h_deg, n_zones, stats);
         hipLaunchKernelGGL(k_layout_decide, dim3(1), dim3(TPB), 0, ctx->stream, stats, n, n_zones,
-                           ra_bits, drop_out);
+                           ra_bits, zone_bits, layout_out);
     });
 }
 

@@ -277,35 +277,43 @@ void scan_exclusive_u32(xm_ctx* ctx, uint32_t* data, uint64_t n);
 constexpr uint64_t SCAN_TILE = 4096;
 inline uint64_t scan_partial_words(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 void scan_exclusive_u32_with(xm_ctx* ctx, uint32_t* data, uint64_t n, uint32_t* partial);
-// Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), in 8-bit digits
-// from lo_bit up; bits from hi_bit up must be equal in all keys.  Buffers are
-// ping-ponged; on return *keys points at the sorted keys and *keys_alt at scratch.
-// digit_totals, when given, holds the counts of every 8-bit digit value at digit
-// positions lo_bit + 8p (256 per position, at least the sort's passes), as
-// k_make_keys counts them while it writes the keys; else the sort reads the keys
-// once to count them.
+// Stable LSD radix sort of 64-bit keys on bits [lo_bit, hi_bit), in digits of
+// digit_bits (8 or 9) bits from lo_bit up; bits from hi_bit up must be equal in all
+// keys.  Buffers are ping-ponged; on return *keys points at the sorted keys and
+// *keys_alt at scratch.  digit_totals, when given, holds the counts of every digit
+// value at digit positions lo_bit + digit_bits p (2^digit_bits per position, at
+// least the sort's passes), as k_make_keys counts them while it writes the keys;
+// else the sort reads the keys once to count them.
 void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit, const uint32_t* digit_totals = nullptr);
+                     int hi_bit, const uint32_t* digit_totals = nullptr, int digit_bits = 8);
 // digit positions of a 64-bit key in 8-bit digits
 constexpr int SORT_MAX_PASSES = 8;
+// 9-bit digits serve keys of at most this many passes: they save a pass over 8-bit
+// ones for keys of 33..36 sorted bits, and 4 x 512 digit counters take no more
+// LDS than 8 x 256 (k_make_keys keeps four blocks a CU)
+constexpr int SORT9_MAX_PASSES = 4;
+// Sort passes of a key of `bits` sorted bits in digits of digit_bits.
+XM_HD int sort_passes(int bits, int digit_bits) { return (bits + digit_bits - 1) / digit_bits; }
 // Largest key count the radix sort takes (32-bit tile indices: the last 8192-key
 // tile must end below 2^32), and so the largest catalog.
 constexpr uint64_t SORT_MAX_KEYS = 0xffffffffull - 8192;
 
-// Adds one key's digits at positions lo_bit + 8p (p < n_pass) to the LDS
-// histogram h[p][256]; the active lanes of a wavefront call it together.  A digit
-// that all 64 lanes hold (segment and zone bits of ordered input, the constant top
-// digits) is added once, by lane 0: 64 atomics on one LDS word would run one after
-// another.
+// Adds one key's digits of DB bits at positions lo_bit + DB p (p < n_pass) to the
+// LDS histogram h[p][2^DB]; the active lanes of a wavefront call it together.  A
+// digit that all 64 lanes hold (segment and zone bits of ordered input, the
+// constant top digits) is added once, by lane 0: 64 atomics on one LDS word would
+// run one after another.
+template <int DB>
 __device__ inline void count_digits(uint32_t* h, uint64_t key, bool valid, int lo_bit,
                                     int n_pass) {
+    constexpr uint32_t R = 1u << DB;
     const bool full = __ballot(valid) == ~uint64_t(0);
     for (int p = 0; p < n_pass; p++) {
-        const uint32_t d = uint32_t(key >> (lo_bit + 8 * p)) & 255u;
+        const uint32_t d = uint32_t(key >> (lo_bit + DB * p)) & (R - 1);
         if (full && __all(d == uint32_t(__builtin_amdgcn_readfirstlane(int(d))))) {
-            if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&h[p * 256 + d], uint32_t(WAVE));
+            if ((threadIdx.x & (WAVE - 1)) == 0) atomicAdd(&h[p * R + d], uint32_t(WAVE));
         } else if (valid) {
-            atomicAdd(&h[p * 256 + d], 1u);
+            atomicAdd(&h[p * R + d], 1u);
         }
     }
 }
@@ -326,24 +334,28 @@ constexpr size_t PIN_JOB_OFFSET = 4096;
 // One-segment build: packed sort words (pack_word) of the records, their zone
 // range and bad count, in one pass.
 // With n_pass > 0 it also adds the counts of every digit value at digit positions
-// lo_bit + 8p (p < n_pass) of the valid records' words into digit_totals[p][256].
-// With ra_drop (device word) the digit positions start ra_drop bits higher and
-// cover only the passes the sort then needs.
+// lo_bit + Dp (p < n_pass) of the valid records' words into digit_totals[p][2^D],
+// D = 8.  With layout (the device word launch_layout writes) the digit width D and
+// the dropped ra bits come from it: the digit positions start that many bits
+// higher and cover only the passes the sort then needs (at most n_pass).
 void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                       PackLayout pl, uint64_t* words, int32_t* zminmax, uint32_t* bad,
                       int lo_bit = 0, int n_pass = 0, uint32_t* digit_totals = nullptr,
-                      const uint32_t* ra_drop = nullptr);
+                      const uint32_t* layout = nullptr);
 // One-segment key layout: from a strided sample of the records, the number of low
-// ra bits the sort and the catalog key drop, written to *drop_out (device; stats is
-// scratch of LAYOUT_STATS_WORDS words, set up by launch_build_init).  TPB must be
-// 256 (one thread per bin).
+// ra bits the sort and the catalog key drop and the sort's digit width, written to
+// *layout_out as drop | digit_bits << 8 (device; stats is scratch of
+// LAYOUT_STATS_WORDS words, set up by launch_build_init).  zone_bits: the key's zone
+// field.  TPB must be 256 (one thread per bin).
 constexpr int LAYOUT_STATS_WORDS = 516;
 void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
-                   int ra_bits, uint32_t* stats, uint32_t* drop_out);
+                   int ra_bits, int zone_bits, uint32_t* stats, uint32_t* layout_out);
+XM_HD int layout_drop(uint32_t layout) { return int(layout & 255u); }
+XM_HD int layout_digit_bits(uint32_t layout) { return (layout >> 8) == 9u ? 9 : 8; }
 // The one-segment build's scratch, BUILD_SMALL_WORDS + LAYOUT_STATS_WORDS +
 // n_digit_words words, set up by one launch: small words {zone min = INT_MAX, zone
-// max = -1, bad records = 0, ra bits dropped = 0}, the layout statistics, the sort's
-// digit counters (zero).
+// max = -1, bad records = 0, layout = 0}, the layout statistics, the sort's digit
+// counters (zero).
 constexpr int BUILD_SMALL_WORDS = 4;
 void launch_build_init(xm_ctx* ctx, uint32_t* scratch, uint64_t n_digit_words);
 // Many-segment build, pass 1: zone range and bad count of every segment

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "xm_internal.h"
 
@@ -197,13 +198,20 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 // cost more traffic than the round trips they save)
 constexpr int LB_WINDOW = 4;
 
-// total[pass][digit] += counts of every digit value over the block's keys (total
-// starts at zero).
+// digit counters of a histogram: every pass of 8-bit digits, or up to
+// SORT9_MAX_PASSES of 9-bit digits
+constexpr int HIST_BINS =
+    SORT_MAX_PASSES * 256 > SORT9_MAX_PASSES * 512 ? SORT_MAX_PASSES * 256 : SORT9_MAX_PASSES * 512;
+
+// total[pass][digit] += counts of every DB-bit digit value over the block's keys
+// (total starts at zero).
+template <int DB>
 __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __restrict__ keys,
                                                            uint32_t n, int lo_bit, int n_pass,
                                                            uint32_t* __restrict__ total) {
-    __shared__ uint32_t h[SORT_MAX_PASSES * 256];
-    for (int i = threadIdx.x; i < n_pass * 256; i += HIST_THREADS) h[i] = 0;
+    constexpr int R = 1 << DB;
+    __shared__ uint32_t h[HIST_BINS];
+    for (int i = threadIdx.x; i < n_pass * R; i += HIST_THREADS) h[i] = 0;
     __syncthreads();
     const uint64_t stride = uint64_t(gridDim.x) * HIST_THREADS * HIST_ITEMS;
     for (uint64_t base = uint64_t(blockIdx.x) * HIST_THREADS * HIST_ITEMS; base < n; base += stride) {
@@ -216,37 +224,44 @@ __global__ __launch_bounds__(HIST_THREADS) void k_radix_hist(const uint64_t* __r
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; j++) {
             const uint64_t i = base + uint64_t(j) * HIST_THREADS + threadIdx.x;
-            count_digits(h, k[j], i < n, lo_bit, n_pass);
+            count_digits<DB>(h, k[j], i < n, lo_bit, n_pass);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n_pass * 256; i += HIST_THREADS)
+    for (int i = threadIdx.x; i < n_pass * R; i += HIST_THREADS)
         if (h[i]) atomicAdd(&total[i], h[i]);
 }
 
 // One block per digit position: start[pass][d] = keys whose digit is below d.
-__global__ __launch_bounds__(256) void k_digit_offsets(const uint32_t* __restrict__ total,
-                                                       uint32_t* __restrict__ start) {
-    __shared__ uint32_t s_w[256 / WAVE + 1];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    start[i] = block_excl_scan<256>(total[i], s_w);
+template <int DB>
+__global__ __launch_bounds__(1 << DB) void k_digit_offsets(const uint32_t* __restrict__ total,
+                                                          uint32_t* __restrict__ start) {
+    constexpr int R = 1 << DB;
+    __shared__ uint32_t s_w[R / WAVE + 1];
+    const uint32_t i = blockIdx.x * R + threadIdx.x;
+    start[i] = block_excl_scan<R>(total[i], s_w);
 }
 
-// One pass over the digit at `shift`.  Tiles take their index from tile_ctr in
-// the order they start, so every tile a look-back waits on is already running.
-// A tile publishes its digit counts (an LDS histogram taken as its keys arrive)
-// before it ranks them, and looks back only after ranking and regrouping, when
-// the earlier tiles have mostly published their prefixes.
-template <typename W>
+// One pass over the DB-bit digit at `shift`.  Tiles take their index from
+// tile_ctr in the order they start, so every tile a look-back waits on is already
+// running.  A tile publishes its digit counts (an LDS histogram taken as its keys
+// arrive) before it ranks them, and looks back only after ranking and regrouping,
+// when the earlier tiles have mostly published their prefixes.  With 9-bit digits
+// the per-wave counts are 16-bit (a wave holds 1024 keys, a tile 8192), so two
+// blocks still fit a CU's LDS.
+template <typename W, int DB>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, uint32_t n, int shift,
     uint32_t parity, const uint32_t* __restrict__ digit_start, W* status, uint32_t* tile_ctr,
     uint32_t* err, uint32_t spin_limit, uint32_t publish) {
     using LB = LbWord<W>;
+    constexpr uint32_t R = 1u << DB;
+    using WC = typename std::conditional<DB == 8, uint32_t, uint16_t>::type;
+    static_assert(R <= uint32_t(RS_THREADS), "one thread per digit value");
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_wc[RS_WAVES][256];   // per-wave running digit counts, then prefixes
-    __shared__ uint32_t s_hist[256];    // the tile's digit counts
-    __shared__ uint32_t s_go[256];      // global start minus tile-local start of each digit
+    __shared__ WC s_wc[RS_WAVES][R];   // per-wave running digit counts, then prefixes
+    __shared__ uint32_t s_hist[R];     // the tile's digit counts
+    __shared__ uint32_t s_go[R];       // global start minus tile-local start of each digit
     __shared__ uint32_t s_w[RS_THREADS / WAVE + 1];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_abort;
@@ -254,8 +269,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int w = tid / WAVE;
-    // threads below 256 own one digit value each in the per-digit steps
-    const bool dig = tid < 256;
+    // threads below R own one digit value each in the per-digit steps
+    const bool dig = uint32_t(tid) < R;
     if (tid == 0) {
         // a failed look-back in an earlier pass left holes of stale words in this
         // pass's input; their digits could place keys past n, so the pass stops
@@ -264,8 +279,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     }
     // per-wave digit masks for the ranking live in s_k until the regroup
     uint64_t* const s_mask = s_k;
-    static_assert(RS_TILE >= RS_WAVES * 256, "s_k holds the ranking masks");
-    for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) {
+    static_assert(RS_TILE >= RS_WAVES * R, "s_k holds the ranking masks");
+    for (uint32_t i = tid; i < RS_WAVES * R; i += RS_THREADS) {
         (&s_wc[0][0])[i] = 0;
         s_mask[i] = 0;
     }
@@ -285,11 +300,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
-        if (i < n) atomicAdd(&s_hist[uint32_t(key[k] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&s_hist[uint32_t(key[k] >> shift) & (R - 1)], 1u);
     }
     __syncthreads();
     const uint32_t cnt = dig ? s_hist[tid] : 0u;
-    W* const mine = status + size_t(tile) * 256 + tid;
+    W* const mine = status + size_t(tile) * R + tid;
     if (dig && publish)
         __hip_atomic_store(mine, LB::make(parity, tile == 0, cnt), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
         const bool valid = i < n;
-        const uint32_t d = uint32_t(key[k] >> shift) & 255u;
+        const uint32_t d = uint32_t(key[k] >> shift) & (R - 1);
         // Valid lanes are a prefix of the wavefront (keys run from the tile start),
         // so lane 0 holds a valid digit whenever any lane does.
         const uint32_t d0 = uint32_t(__builtin_amdgcn_readfirstlane(int(d)));
@@ -308,7 +323,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
             const uint64_t m = __ballot(valid);
             const uint32_t pre = s_wc[w][d0];
             rk[k] = pre + uint32_t(__popcll(m & lt));
-            if (valid && (m >> lane) == 1u) s_wc[w][d0] = pre + uint32_t(__popcll(m));
+            if (valid && (m >> lane) == 1u) s_wc[w][d0] = WC(pre + uint32_t(__popcll(m)));
             continue;
         }
         // lanes holding the same digit: each sets its bit in the digit's mask, then
@@ -316,17 +331,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         uint64_t m = 0;
         uint32_t pre = 0;
         if (valid) {
-            atomicOr(reinterpret_cast<unsigned long long*>(&s_mask[w * 256 + d]),
+            atomicOr(reinterpret_cast<unsigned long long*>(&s_mask[w * R + d]),
                      (unsigned long long)(uint64_t(1) << lane));
-            m = s_mask[w * 256 + d];
+            m = s_mask[w * R + d];
             pre = s_wc[w][d];
         }
         rk[k] = pre + uint32_t(__popcll(m & lt));
         // the highest lane of each digit group advances that digit's count and
         // clears its mask, after the wavefront's reads above
         if (valid && (m >> lane) == 1u) {
-            s_wc[w][d] = pre + uint32_t(__popcll(m));
-            s_mask[w * 256 + d] = 0;
+            s_wc[w][d] = WC(pre + uint32_t(__popcll(m)));
+            s_mask[w * R + d] = 0;
         }
     }
     // tile-local start of each digit; then s_wc[q][d] becomes the tile-local
@@ -338,7 +353,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 #pragma unroll
         for (int q = 0; q < RS_WAVES; q++) {
             const uint32_t c = s_wc[q][tid];
-            s_wc[q][tid] = run;
+            s_wc[q][tid] = WC(run);
             run += c;
         }
     }
@@ -348,7 +363,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     for (int k = 0; k < RS_ITEMS; k++) {
         const uint32_t i = base + k * WAVE + lane;
         if (i < n) {
-            const uint32_t d = uint32_t(key[k] >> shift) & 255u;
+            const uint32_t d = uint32_t(key[k] >> shift) & (R - 1);
             s_k[s_wc[w][d] + rk[k]] = key[k];
         }
     }
@@ -363,7 +378,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 #pragma unroll
             for (int k = 0; k < LB_WINDOW; k++) {
                 // tile 0 always holds a prefix, so the walk never passes it
-                v[k] = j - k >= 0 ? __hip_atomic_load(status + size_t(j - k) * 256 + tid,
+                v[k] = j - k >= 0 ? __hip_atomic_load(status + size_t(j - k) * R + tid,
                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                   : LB::make(parity, true, 0);
             }
@@ -397,22 +412,24 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t tile_n = (n - tile0) < uint32_t(RS_TILE) ? (n - tile0) : uint32_t(RS_TILE);
     for (uint32_t p = tid; p < tile_n; p += RS_THREADS) {
         const uint64_t kk = s_k[p];
-        kout[s_go[uint32_t(kk >> shift) & 255u] + p] = kk;
+        kout[s_go[uint32_t(kk >> shift) & (R - 1)] + p] = kk;
     }
 }
 
-template <typename W>
+template <typename W, int DB>
 void scatter_passes(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
                     int n_pass, uint32_t nblocks, const uint32_t* start, void* status,
                     uint32_t* ctr) {
     // a look-back that reaches its polling limit sets the context's sort error
     // word (reported by sort_check)
     uint32_t* const err = sort_error_word(ctx);
+    const auto kernel = k_radix_scatter<W, DB>;
     for (int p = 0; p < n_pass; p++) {
         launch_timed(ctx, "radix_scatter", [&] {
-            hipLaunchKernelGGL(k_radix_scatter<W>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream,
-                               (const uint64_t*)*keys, *keys_alt, uint32_t(n), lo_bit + 8 * p,
-                               uint32_t(p + 1), start + p * 256, static_cast<W*>(status),
+            hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(RS_THREADS), 0,
+                               ctx->stream, (const uint64_t*)*keys, *keys_alt, uint32_t(n),
+                               lo_bit + DB * p, uint32_t(p + 1), start + p * (1 << DB),
+                               static_cast<W*>(status),
                                ctr + p, err,
                                // a context set to fail its sorts (tests of the error
                                // path): tiles publish nothing and poll 64 times
@@ -471,25 +488,25 @@ uint32_t* sort_error_word(xm_ctx* ctx) {
     return ctx->sort_err;
 }
 
-void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
-                     int hi_bit, const uint32_t* digit_totals) {
-    if (n <= 1 || hi_bit <= lo_bit) return;
-    static_assert(SORT_MAX_KEYS == 0xffffffffull - RS_TILE, "the key limit follows the tile size");
-    if (n > SORT_MAX_KEYS)
-        throw InvalidArg{"radix sort supports up to 2^32 - " + std::to_string(RS_TILE + 1) + " keys"};
-    if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
-    // 8-bit digits starting at lo_bit (the last one may reach past hi_bit into bits
-    // that are equal in every key)
-    const int n_pass = (hi_bit - lo_bit + 7) / 8;
+namespace {
+
+template <int DB>
+void radix_sort_db(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
+                   int hi_bit, const uint32_t* digit_totals) {
+    constexpr int R = 1 << DB;
+    // digits starting at lo_bit (the last one may reach past hi_bit into bits that
+    // are equal in every key)
+    const int n_pass = sort_passes(hi_bit - lo_bit, DB);
+    if (n_pass * R > HIST_BINS) throw InvalidArg{"radix sort: too many digit passes"};
     const uint32_t nblocks = uint32_t((n + RS_TILE - 1) / RS_TILE);
     const uint32_t hist_blocks = std::min<uint32_t>(
         HIST_BLOCKS, uint32_t((n + HIST_THREADS * HIST_ITEMS - 1) / (HIST_THREADS * HIST_ITEMS)));
     // one block: look-back words | tile counters | digit totals | digit starts
     // 32-bit look-back words suffice below 2^30 keys
     const bool narrow = n < (uint64_t(1) << 30) && !ctx->sort_wide_words;
-    const size_t status_bytes = size_t(nblocks) * 256 * (narrow ? 4 : 8);
+    const size_t status_bytes = size_t(nblocks) * R * (narrow ? 4 : 8);
     const size_t ctr_words = 64;   // SORT_MAX_PASSES tile counters, padded
-    const size_t digit_words = size_t(n_pass) * 256;
+    const size_t digit_words = size_t(n_pass) * R;
     ScopedBlocks tmp(ctx);
     char* blk = tmp.alloc<char>(status_bytes + (ctr_words + 2 * digit_words) * 4);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(blk + status_bytes);
@@ -499,18 +516,38 @@ void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t
                     status_bytes / 4 + ctr_words + (digit_totals ? 0 : digit_words), 0u);
     if (!digit_totals) {
         launch_timed(ctx, "radix_hist", [&] {
-            hipLaunchKernelGGL(k_radix_hist, dim3(hist_blocks), dim3(HIST_THREADS), 0, ctx->stream,
-                               (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass, total);
+            hipLaunchKernelGGL(k_radix_hist<DB>, dim3(hist_blocks), dim3(HIST_THREADS), 0,
+                               ctx->stream, (const uint64_t*)*keys, uint32_t(n), lo_bit, n_pass,
+                               total);
         });
     }
     launch_timed(ctx, "radix_offsets", [&] {
-        hipLaunchKernelGGL(k_digit_offsets, dim3(n_pass), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_digit_offsets<DB>, dim3(n_pass), dim3(R), 0, ctx->stream,
                            digit_totals ? digit_totals : (const uint32_t*)total, start);
     });
     if (narrow)
-        scatter_passes<uint32_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
+        scatter_passes<uint32_t, DB>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk,
+                                     ctr);
     else
-        scatter_passes<uint64_t>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk, ctr);
+        scatter_passes<uint64_t, DB>(ctx, keys, keys_alt, n, lo_bit, n_pass, nblocks, start, blk,
+                                     ctr);
+}
+
+}  // namespace
+
+void radix_sort_keys(xm_ctx* ctx, uint64_t** keys, uint64_t** keys_alt, uint64_t n, int lo_bit,
+                     int hi_bit, const uint32_t* digit_totals, int digit_bits) {
+    if (n <= 1 || hi_bit <= lo_bit) return;
+    static_assert(SORT_MAX_KEYS == 0xffffffffull - RS_TILE, "the key limit follows the tile size");
+    if (n > SORT_MAX_KEYS)
+        throw InvalidArg{"radix sort supports up to 2^32 - " + std::to_string(RS_TILE + 1) + " keys"};
+    if (lo_bit < 0 || hi_bit > 64) throw InvalidArg{"radix sort bit range outside the key"};
+    if (digit_bits == 8)
+        radix_sort_db<8>(ctx, keys, keys_alt, n, lo_bit, hi_bit, digit_totals);
+    else if (digit_bits == 9)
+        radix_sort_db<9>(ctx, keys, keys_alt, n, lo_bit, hi_bit, digit_totals);
+    else
+        throw InvalidArg{"radix sort digits are 8 or 9 bits"};
 }
 
 void sort_check(xm_ctx* ctx) {
