@@ -55,15 +55,21 @@ BYTES_PER_SOURCE = {
     "radix_scatter": 16.0,   # read packed word 8, write packed word 8
     "radix_hist": 8.0,       # read packed word (once per sort, every digit)
     "make_keys": 32.0,       # read 24-B record, write packed word 8
+    # catalogs of 2^26 rows and more: the key pass also writes each record into its
+    # zone bucket: read 24, write 24 + 8
+    "make_keys_bucketed": 56.0,
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
     "gather_fp32": 88.0,     # the same + the 16-B FP32 row
 }
-KERNELS = ["layout", "make_keys", "radix_hist", "radix_offsets", "radix_scatter", "scan_reduce",
-           "scan_top", "scan_apply", "gather", "chunk_counts", "chunk_fill", "fill",
-           "pair_fp64", "pair_fp32", "region_scan", "compact", "job_prep"]
+KERNELS = ["layout", "bucket_plan", "make_keys", "radix_hist", "radix_offsets",
+           "radix_scatter", "scan_reduce", "scan_top", "scan_apply", "gather", "chunk_counts",
+           "chunk_fill", "fill", "pair_fp64", "pair_fp32", "region_scan", "compact", "job_prep"]
 
 # rocprofv3 names of kernels whose timing name differs
 PROFILE_NAMES = {"pair_fp32": "pair<Fp32Cvt>", "pair_fp64": "pair<Fp64>"}
+# the library's default bucket_min_rows: one-segment catalogs from this size up are
+# built with their records in zone buckets
+BUCKET_MIN_ROWS = 1 << 26
 
 PLANE_SEED = 2003  # device generator key of the headline catalogs
 # the forms of the headline join: (name, precision, fp32_rows, --mode key).  The FP32
@@ -337,7 +343,9 @@ def run_headline(args):
     f32s = modes["fp32_filter_stored_fp32_rows"]
     # the dominant kernel of the FP32 form: its algorithmic bytes per launch over
     # its average launch in the timed steps
-    if dom in BYTES_PER_SOURCE:
+    if dom == "make_keys" and n >= BUCKET_MIN_ROWS:
+        alg = BYTES_PER_SOURCE["make_keys_bucketed"] * n
+    elif dom in BYTES_PER_SOURCE:
         alg = BYTES_PER_SOURCE[dom] * n
     elif dom == "pair_fp32":
         alg = f32["pair_roofline"]["algorithmic_bytes_per_launch"]

@@ -558,6 +558,43 @@ def test_fp32_rows_and_fp64_rows_rounded_in_registers_agree(gpu_ctx):
     assert st_cvt["max_dist_err_arcsec"] == st_rows["max_dist_err_arcsec"] > 0
 
 
+@pytest.mark.parametrize("pct", [115, 70, 10])
+def test_bucketed_build_equals_plain(gpu_ctx, pct):
+    # the key pass that moves the records into zone buckets, forced at this size:
+    # with runs of 115% of the sampled estimate (the default), 70% (records spill
+    # into the overflow run) and 10% (the overflow run fills too and the build falls
+    # back to the plain one) the catalog is the plain build's row for row, and so
+    # are the joins
+    A, B, r = synth.config_small_patch(seed=71, n_a=300_000, n_b=300_000)
+    cp, cbp = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
+    ep = cp.export()
+    res = gpu_ctx.match(cp, cbp, r)
+    ref = res.to_host()
+    res.close(); cp.close(); cbp.close()
+    gpu_ctx.set_param("bucket_min_rows", 1)
+    gpu_ctx.set_param("bucket_capacity_pct", pct)
+    try:
+        assert gpu_ctx.get_param("bucket_capacity_pct") == pct
+        ca, cb = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
+        eb = ca.export()
+        res = gpu_ctx.match(ca, cb, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+        got = res.to_host()
+        res.close(); ca.close(); cb.close()
+    finally:
+        gpu_ctx.set_param("bucket_min_rows", 2**26)
+        gpu_ctx.set_param("bucket_capacity_pct", 115)
+    for f in ep:
+        assert np.array_equal(ep[f], eb[f]), f
+    assert_same(got, ref)
+
+
+def test_bucket_knobs_reject_bad_values(gpu_ctx):
+    for name, v in (("bucket_min_rows", -1), ("bucket_capacity_pct", 0),
+                    ("bucket_capacity_pct", 1001)):
+        with pytest.raises(xm.XmatchError):
+            gpu_ctx.set_param(name, v)
+
+
 def test_autotune_keeps_the_sets(gpu_ctx):
     # a batch of jobs of different densities and radii, with and without the
     # tuner: the same pairs per job; the tune table names the measured keys

@@ -345,12 +345,16 @@ void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
 }
 
 // One segment: a single pass over the records packs zone | ra | row words with the
-// absolute zone (the zone range comes back with the same pass).
-void build_one(xm_ctx* ctx, xm_catalog* cat) {
+// absolute zone (the zone range comes back with the same pass).  A catalog of at
+// least bucket_min_rows rows has that pass also move its records into zone buckets
+// (BucketBuild), which the gather then reads from one XCD's L2 each; returns false,
+// having built nothing, when the buckets' sampled plan failed.
+bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
     SegDesc& d = cat->seg[0];
     const uint64_t n = d.n;
-    if (n == 0) return;
-    PackLayout pl = pack_layout(n, d.n_zones);
+    const BucketBuild bb = bucket_build(n, ctx->bucket_capacity_pct);
+    // the row field holds the input row, or the record's place among the buckets
+    PackLayout pl = pack_layout(bucketed ? bb.c_total : n, d.n_zones);
     if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
     d.idx_bits = pl.idx_bits;
     d.zone_bits = pl.zone_bits;
@@ -375,23 +379,43 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     // the device from a sample
     launch_layout(ctx, d.recs, n, d.h_deg, d.n_zones, pl.ra_bits, pl.zone_bits, stats,
                   reinterpret_cast<uint32_t*>(small + 3));
-    launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
-                     reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
-                     digit_totals, reinterpret_cast<uint32_t*>(small + 3));
+    xm_obj* part = nullptr;
+    uint32_t* plan = nullptr;
+    const size_t runs = size_t(bb.n_buckets) * BUCKET_GROUPS;
+    if (bucketed) {
+        part = tmp.alloc<xm_obj>(bb.c_total * sizeof(xm_obj));
+        plan = tmp.alloc<uint32_t>((2 * runs + BP_WORDS + bb.n_buckets) * sizeof(uint32_t));
+        uint32_t* hist = plan + 2 * runs + BP_WORDS;
+        launch_fill_u32(ctx, hist, bb.n_buckets, 0u);
+        launch_bucket_plan(ctx, d.recs, n, d.h_deg, d.n_zones, stats, bb,
+                           ctx->bucket_capacity_pct, hist, plan);
+        launch_make_keys_bucketed(ctx, d.recs, n, d.h_deg, d.n_zones, pl, stats, bb, plan, part,
+                                  words, small, reinterpret_cast<uint32_t*>(small + 2),
+                                  pl.idx_bits, hist_pass, digit_totals,
+                                  reinterpret_cast<uint32_t*>(small + 3));
+    } else {
+        launch_make_keys(ctx, d.recs, n, d.h_deg, d.n_zones, pl, words, small,
+                         reinterpret_cast<uint32_t*>(small + 2), pl.idx_bits, hist_pass,
+                         digit_totals, reinterpret_cast<uint32_t*>(small + 3));
+    }
     int32_t* pin = static_cast<int32_t*>(ctx->pinned);
     XM_HIP(hipMemcpyAsync(pin, small, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (bucketed)
+        XM_HIP(hipMemcpyAsync(pin + 4, plan + 2 * runs + BP_FAILED, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
     const int32_t zmin = pin[0], zmax = pin[1];
     const uint32_t nbad = uint32_t(pin[2]);
     const int ra_drop = layout_drop(uint32_t(pin[3]));
     const int digit_bits = layout_digit_bits(uint32_t(pin[3]));
-    d.ra_bits = pl.ra_bits - ra_drop;
-    d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     if (nbad) {
         throw InvalidArg{std::to_string(nbad) +
                          " record(s) with NaN or out-of-range coordinates "
                          "(need 0 <= ra < 360, -90 <= dec <= 90)"};
     }
+    if (bucketed && pin[4]) return false;
+    d.ra_bits = pl.ra_bits - ra_drop;
+    d.ra_gran = int64_t(1) << (RA_BITS - d.ra_bits);
     d.zmin = zmin;
     d.zmax = zmax;
     // Sort on the ra bits and the zone bits that differ between zmin and zmax (the
@@ -405,11 +429,22 @@ void build_one(xm_ctx* ctx, xm_catalog* cat) {
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     if (ctx->fp32_rows) cat->row32 = dalloc(ctx, n * sizeof(Row32));
-    launch_gather(ctx, d.recs, words, n, pl, ra_drop, zmin, static_cast<Row64*>(cat->row),
-                  static_cast<Row32*>(cat->row32), cat->zoff, cat->n_zoff, words_alt);
+    launch_gather(ctx, bucketed ? part : d.recs, words, n, pl, ra_drop, zmin,
+                  static_cast<Row64*>(cat->row), static_cast<Row32*>(cat->row32), cat->zoff,
+                  cat->n_zoff, words_alt, bucketed);
     tmp.free(words);
     tmp.keep(words_alt);
     cat->key = words_alt;
+    return true;
+}
+
+void build_one(xm_ctx* ctx, xm_catalog* cat) {
+    const uint64_t n = cat->seg[0].n;
+    if (n == 0) return;
+    const bool bucketed = ctx->bucket_min_rows && n >= ctx->bucket_min_rows &&
+                          n <= BUCKET_MAX_ROWS;
+    if (bucketed && build_one_pass(ctx, cat, true)) return;
+    build_one_pass(ctx, cat, false);
 }
 
 // Many segments: pass 1 finds every segment's zone range, the host picks each
@@ -662,6 +697,13 @@ xm_status xm_ctx_set_param(xm_ctx* ctx, const char* name, int64_t value) {
         } else if (n == "sort_wide_words") {
             if (value != 0 && value != 1) throw InvalidArg{"sort_wide_words must be 0 or 1"};
             ctx->sort_wide_words = value != 0;
+        } else if (n == "bucket_min_rows") {
+            if (value < 0) throw InvalidArg{"bucket_min_rows must be >= 0"};
+            ctx->bucket_min_rows = uint64_t(value);
+        } else if (n == "bucket_capacity_pct") {
+            if (value < 1 || value > 1000)
+                throw InvalidArg{"bucket_capacity_pct must be in [1, 1000]"};
+            ctx->bucket_capacity_pct = uint32_t(value);
         } else if (n == "sort_fail_test") {
             if (value != 0 && value != 1) throw InvalidArg{"sort_fail_test must be 0 or 1"};
             ctx->sort_fail_test = value != 0;
@@ -689,6 +731,8 @@ xm_status xm_ctx_get_param(xm_ctx* ctx, const char* name, int64_t* value) {
         if (n == "dense_min") *value = ctx->dense_min;
         else if (n == "key_ra_bits") *value = ctx->key_ra_bits;
         else if (n == "sort_wide_words") *value = ctx->sort_wide_words ? 1 : 0;
+        else if (n == "bucket_min_rows") *value = int64_t(ctx->bucket_min_rows);
+        else if (n == "bucket_capacity_pct") *value = ctx->bucket_capacity_pct;
         else if (n == "sort_fail_test") *value = ctx->sort_fail_test ? 1 : 0;
         else if (n == "fp32_rows") *value = ctx->fp32_rows ? 1 : 0;
         else if (n == "chunk_rows") *value = ctx->chunk_rows ? ctx->chunk_rows : CHUNK_ROWS;

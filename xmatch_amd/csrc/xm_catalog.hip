@@ -445,8 +445,15 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
                                                 Row32* __restrict__ rows32,
                                                 uint32_t* __restrict__ zoff, uint32_t n_zoff,
                                                 uint64_t* __restrict__ keys,
-                                                const uint32_t* __restrict__ sort_err) {
-    const uint32_t i = blockIdx.x * GATHER_TPB + threadIdx.x;
+                                                const uint32_t* __restrict__ sort_err,
+                                                uint32_t xcd_runs) {
+    uint32_t blk = blockIdx.x;
+    if (xcd_runs) {
+        // blocks are dealt to the XCDs round-robin: XCD x takes the x-th run of rows
+        const uint32_t per = gridDim.x / 8;
+        if (blk < per * 8) blk = (blk % 8) * per + blk / 8;
+    }
+    const uint32_t i = blk * GATHER_TPB + threadIdx.x;
     if (i >= n || *sort_err) return;
     const uint64_t wd = words[i];
     const int zs = pl.idx_bits + pl.ra_bits;
@@ -537,6 +544,294 @@ inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 // each: small = {INT_MAX, -1, 0, 0} (zone min, zone max, bad records, layout),
 // then the layout statistics (zero but zone min INT_MAX and zone max -1),
 // then the digit counters (zero).
+// ---- one segment, records in zone buckets
+// Bucket of zone z: the sampled zone range [zlo, zhi] (layout statistics) cut into
+// n_buckets equal parts; zones outside it go to the first or last bucket.
+__device__ inline void bucket_range(const uint32_t* __restrict__ stats, int32_t n_zones,
+                                    int32_t* zlo, int64_t* span) {
+    int32_t a = int32_t(stats[512]), b = int32_t(stats[513]);
+    if (b < a) {  // nothing valid sampled
+        a = 0;
+        b = n_zones - 1;
+    }
+    *zlo = a;
+    *span = int64_t(b) - int64_t(a) + 1;
+}
+
+__device__ inline uint32_t bucket_of(int32_t z, int32_t zlo, int64_t span, uint32_t nb) {
+    const int64_t k = (int64_t(z) - int64_t(zlo)) * int64_t(nb) / span;
+    return uint32_t(k < 0 ? 0 : (k >= int64_t(nb) ? int64_t(nb) - 1 : k));
+}
+
+// hist[b] += sampled records of bucket b (m records at an even stride)
+__global__ __launch_bounds__(TPB) void k_bucket_sample(const xm_obj* __restrict__ recs, uint64_t n,
+                                                       uint32_t m, double h_deg, int32_t n_zones,
+                                                       const uint32_t* __restrict__ stats,
+                                                       uint32_t nb, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[MAX_BUCKETS];
+    for (uint32_t b = threadIdx.x; b < nb; b += TPB) h[b] = 0;
+    __syncthreads();
+    int32_t zlo;
+    int64_t span;
+    bucket_range(stats, n_zones, &zlo, &span);
+    for (uint32_t t = blockIdx.x * TPB + threadIdx.x; t < m; t += gridDim.x * TPB) {
+        const uint64_t i = uint64_t(t) * n / m;
+        const double ra = recs[i].ra_deg;
+        const double dec = recs[i].dec_deg;
+        if (!valid_record(ra, dec)) continue;
+        atomicAdd(&h[bucket_of(zone_of(dec, h_deg, n_zones), zlo, span, nb)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += TPB)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+// One block.  Run (b, g) -- bucket b, group g -- holds est(b) pct / 100 / groups +
+// BUCKET_SLACK rows, est(b) = n hist[b] / sampled; runs are laid out bucket by bucket
+// (a bucket's groups side by side); cursor[g][b] = the run's start, end[g][b] its end,
+// the overflow cursor starts after the last run.
+constexpr uint32_t PLAN_TPB = 1024;
+__global__ __launch_bounds__(PLAN_TPB) void k_bucket_plan(const uint32_t* __restrict__ hist,
+                                                          uint32_t nb, uint64_t n, uint32_t pct,
+                                                          uint32_t* __restrict__ plan) {
+    __shared__ uint64_t s_part[PLAN_TPB];
+    __shared__ uint64_t s_tot;
+    const uint32_t t = threadIdx.x;
+    uint64_t sum = 0;
+    for (uint32_t b = t; b < nb; b += PLAN_TPB) sum += hist[b];
+    s_part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint64_t a = 0;
+        for (uint32_t k = 0; k < PLAN_TPB; k++) a += s_part[k];
+        s_tot = a;
+    }
+    __syncthreads();
+    const uint64_t sampled = s_tot;
+    const uint32_t runs = nb * BUCKET_GROUPS;
+    auto run_rows = [&](uint32_t e) -> uint64_t {
+        const uint32_t b = e / BUCKET_GROUPS;
+        const uint64_t est = sampled ? uint64_t(hist[b]) * n / sampled : n / nb;
+        return est * pct / (100 * BUCKET_GROUPS) + BUCKET_SLACK;
+    };
+    // each thread a contiguous range of runs; a serial scan of the thread sums
+    const uint32_t per = (runs + PLAN_TPB - 1) / PLAN_TPB;
+    const uint32_t e0 = t * per, e1 = e0 + per < runs ? e0 + per : runs;
+    uint64_t mine = 0;
+    for (uint32_t e = e0; e < e1; e++) mine += run_rows(e);
+    __syncthreads();
+    s_part[t] = mine;
+    __syncthreads();
+    if (t == 0) {
+        uint64_t a = 0;
+        for (uint32_t k = 0; k < PLAN_TPB; k++) {
+            const uint64_t v = s_part[k];
+            s_part[k] = a;
+            a += v;
+        }
+        s_tot = a;
+    }
+    __syncthreads();
+    uint64_t run = s_part[t];
+    uint32_t* const cursor = plan;
+    uint32_t* const end = plan + runs;
+    for (uint32_t e = e0; e < e1; e++) {
+        const uint32_t b = e / BUCKET_GROUPS, g = e % BUCKET_GROUPS;
+        cursor[g * nb + b] = uint32_t(run);
+        run += run_rows(e);
+        end[g * nb + b] = uint32_t(run);
+    }
+    if (t == 0) {
+        plan[2 * runs + BP_OVF_CURSOR] = uint32_t(s_tot);
+        plan[2 * runs + BP_FAILED] = 0;
+    }
+}
+
+// The key pass of a bucketed build.  Block k takes rows [k per_blk, (k + 1) per_blk)
+// in chunks of BK_CHUNK and appends to the runs of group k % BUCKET_GROUPS: the
+// chunk's records are counted per bucket in LDS, each bucket reserves its count in
+// its run with one atomic (the rest, past the run's end, in the overflow run), and
+// the records are regrouped by bucket in LDS and written as contiguous runs.  The
+// words go out in input order with the row field = the record's place in part, so
+// the sort orders ties by input row as the plain build does, and the catalog is the
+// same.
+constexpr uint32_t BK_TPB = 1024;
+constexpr uint32_t BK_CHUNK = 4096;
+constexpr uint32_t BK_ITEMS = BK_CHUNK / BK_TPB;
+constexpr uint32_t BK_NONE = 0xffffffffu;
+__global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
+    const xm_obj* __restrict__ recs, uint32_t n, uint32_t per_blk, double h_deg,
+    int32_t n_zones, PackLayout pl, const uint32_t* __restrict__ stats, uint32_t nb,
+    uint32_t* __restrict__ plan, uint32_t ovf_end, xm_obj* __restrict__ part,
+    uint64_t* __restrict__ words, int32_t* __restrict__ zminmax, uint32_t* __restrict__ bad,
+    int lo_bit, int n_pass, uint32_t* __restrict__ digit_totals,
+    const uint32_t* __restrict__ layout) {
+    __shared__ xm_obj s_rec[BK_CHUNK];
+    __shared__ uint32_t s_pos[BK_CHUNK];
+    __shared__ uint32_t s_lb[MAX_BUCKETS + 1];  // counts, then chunk-local starts
+    __shared__ uint32_t s_base[MAX_BUCKETS];    // the run start each bucket reserved
+    __shared__ uint32_t s_fit[MAX_BUCKETS];     // how many of them fit the run
+    __shared__ uint32_t s_obase[MAX_BUCKETS];   // overflow start of the rest
+    __shared__ uint32_t s_h[MAKE_KEYS_HIST_BINS];
+    __shared__ uint32_t s_part[BK_TPB / WAVE];
+    __shared__ int32_t s_min[BK_TPB / WAVE], s_max[BK_TPB / WAVE];
+    __shared__ uint32_t s_bad[BK_TPB / WAVE];
+
+    int db = 8;
+    if (layout) {
+        const uint32_t lw = *layout;
+        const int drop = layout_drop(lw);
+        db = layout_digit_bits(lw);
+        lo_bit += drop;
+        const int need = sort_passes(pl.ra_bits - drop + pl.zone_bits, db);
+        n_pass = n_pass < need ? n_pass : need;
+    }
+    const int bins = n_pass << db;
+    for (int i = threadIdx.x; i < bins; i += BK_TPB) s_h[i] = 0;
+    int32_t zlo;
+    int64_t span;
+    bucket_range(stats, n_zones, &zlo, &span);
+    const uint32_t runs = nb * BUCKET_GROUPS;
+    const uint32_t g = blockIdx.x % BUCKET_GROUPS;
+    uint32_t* const cursor = plan + g * nb;
+    const uint32_t* const run_end = plan + runs + g * nb;
+    uint32_t* const ovf = plan + 2 * runs;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    int32_t zmn = 0x7fffffff, zmx = -1;
+    uint32_t nbad = 0;
+    const uint64_t lo = uint64_t(blockIdx.x) * per_blk;
+    const uint64_t hi = lo + per_blk < n ? lo + per_blk : n;
+    for (uint64_t c0 = lo; c0 < hi; c0 += BK_CHUNK) {
+        const uint32_t m = uint32_t(hi - c0 < BK_CHUNK ? hi - c0 : BK_CHUNK);
+        for (uint32_t b = threadIdx.x; b <= nb; b += BK_TPB) s_lb[b] = 0;
+        __syncthreads();
+        xm_obj x[BK_ITEMS];
+        uint64_t wd[BK_ITEMS];
+        uint32_t bk[BK_ITEMS], rk[BK_ITEMS];
+#pragma unroll
+        for (uint32_t j = 0; j < BK_ITEMS; j++) {
+            const uint32_t sl = j * BK_TPB + threadIdx.x;
+            bk[j] = BK_NONE;
+            wd[j] = ~uint64_t(0);
+            bool valid = false;
+            if (sl < m) {
+                x[j] = recs[c0 + sl];
+                valid = valid_record(x[j].ra_deg, x[j].dec_deg);
+                if (valid) {
+                    const int32_t z = zone_of(x[j].dec_deg, h_deg, n_zones);
+                    wd[j] = pack_word(pl, z, x[j].ra_deg, 0);
+                    zmn = min(zmn, z);
+                    zmx = max(zmx, z);
+                    bk[j] = bucket_of(z, zlo, span, nb);
+                    rk[j] = atomicAdd(&s_lb[bk[j]], 1u);
+                } else {
+                    nbad++;
+                }
+            }
+            if (db == 9) count_digits<9>(s_h, wd[j], valid, lo_bit, n_pass);
+            else count_digits<8>(s_h, wd[j], valid, lo_bit, n_pass);
+        }
+        __syncthreads();
+        // exclusive scan of the bucket counts (two per thread), s_lb[nb] = total
+        {
+            const uint32_t b0 = 2 * threadIdx.x;
+            const uint32_t v0 = b0 < nb ? s_lb[b0] : 0u;
+            const uint32_t v1 = b0 + 1 < nb ? s_lb[b0 + 1] : 0u;
+            uint32_t inc = v0 + v1;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t o = __shfl_up(inc, d, WAVE);
+                if (lane >= d) inc += o;
+            }
+            if (lane == WAVE - 1) s_part[w] = inc;
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (int q = 0; q < w; q++) wbase += s_part[q];
+            const uint32_t ex = wbase + inc - v0 - v1;
+            __syncthreads();
+            if (b0 < nb) s_lb[b0] = ex;
+            if (b0 + 1 < nb) s_lb[b0 + 1] = ex + v0;
+            if (threadIdx.x == BK_TPB - 1) {
+                uint32_t tot = 0;
+                for (int q = 0; q < BK_TPB / WAVE; q++) tot += s_part[q];
+                s_lb[nb] = tot;
+            }
+        }
+        __syncthreads();
+        // each bucket reserves its rows in its run, the rest in the overflow run
+        for (uint32_t b = threadIdx.x; b < nb; b += BK_TPB) {
+            const uint32_t cnt = s_lb[b + 1] - s_lb[b];
+            if (!cnt) continue;
+            const uint32_t base = atomicAdd(&cursor[b], cnt);
+            const uint32_t e = run_end[b];
+            const uint32_t fit = base >= e ? 0u : (e - base < cnt ? e - base : cnt);
+            s_base[b] = base;
+            s_fit[b] = fit;
+            if (fit < cnt) {
+                const uint32_t ob = atomicAdd(&ovf[BP_OVF_CURSOR], cnt - fit);
+                if (uint64_t(ob) + (cnt - fit) > ovf_end) {
+                    atomicOr(&ovf[BP_FAILED], 1u);
+                    s_obase[b] = BK_NONE;
+                } else {
+                    s_obase[b] = ob;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < BK_ITEMS; j++) {
+            const uint32_t sl = j * BK_TPB + threadIdx.x;
+            if (sl >= m) continue;
+            uint32_t pos = BK_NONE;
+            if (bk[j] != BK_NONE) {
+                const uint32_t b = bk[j];
+                if (rk[j] < s_fit[b]) pos = s_base[b] + rk[j];
+                else if (s_obase[b] != BK_NONE) pos = s_obase[b] + (rk[j] - s_fit[b]);
+                const uint32_t slot = s_lb[b] + rk[j];
+                s_rec[slot] = x[j];
+                s_pos[slot] = pos;
+            }
+            words[c0 + sl] = pos != BK_NONE ? (wd[j] | pos) : ~uint64_t(0);
+        }
+        __syncthreads();
+        const uint32_t staged = s_lb[nb];
+        for (uint32_t sl = threadIdx.x; sl < staged; sl += BK_TPB) {
+            const uint32_t pos = s_pos[sl];
+            if (pos != BK_NONE) part[pos] = s_rec[sl];
+        }
+        __syncthreads();
+    }
+    // zone range and bad count: one atomic each per block
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        zmn = min(zmn, __shfl_xor(zmn, d, WAVE));
+        zmx = max(zmx, __shfl_xor(zmx, d, WAVE));
+        nbad += __shfl_xor(nbad, d, WAVE);
+    }
+    if (lane == 0) {
+        s_min[w] = zmn;
+        s_max[w] = zmx;
+        s_bad[w] = nbad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t a = s_min[0], b = s_max[0];
+        uint32_t nbb = s_bad[0];
+        for (int q = 1; q < BK_TPB / WAVE; q++) {
+            a = min(a, s_min[q]);
+            b = max(b, s_max[q]);
+            nbb += s_bad[q];
+        }
+        if (b >= 0) {
+            atomicMin(&zminmax[0], a);
+            atomicMax(&zminmax[1], b);
+        }
+        if (nbb) atomicAdd(bad, nbb);
+    }
+    for (int i = threadIdx.x; i < bins; i += BK_TPB)
+        if (s_h[i]) atomicAdd(&digit_totals[i], s_h[i]);
+}
+
 __global__ __launch_bounds__(TPB) void k_build_init(uint32_t* __restrict__ w, uint32_t n_words) {
     const uint32_t i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n_words) return;
@@ -585,14 +880,14 @@ void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, in
 
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
                    PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
-                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys) {
+                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys, bool xcd_runs) {
     if (n == 0) return;
     const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
         hipLaunchKernelGGL(k_gather, dim3(uint32_t((n + GATHER_TPB - 1) / GATHER_TPB)),
                            dim3(GATHER_TPB), 0, ctx->stream, recs, words,
                            uint32_t(n), pl, ra_drop, zmin, rows, rows32, zoff, uint32_t(n_zoff),
-                           keys, err);
+                           keys, err, uint32_t(xcd_runs ? 1 : 0));
     });
 }
 
@@ -664,6 +959,52 @@ void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v) {
     if (n == 0) return;
     launch_timed(ctx, "fill", [&] {
         hipLaunchKernelGGL(k_fill<int32_t>, grid_for(n), dim3(TPB), 0, ctx->stream, p, n, v);
+    });
+}
+
+BucketBuild bucket_build(uint64_t n, uint32_t capacity_pct) {
+    BucketBuild bb;
+    // ~2 MB of records a bucket, in [64, MAX_BUCKETS]
+    const uint64_t want = n * sizeof(xm_obj) / (uint64_t(2) << 20);
+    bb.n_buckets = uint32_t(std::min<uint64_t>(MAX_BUCKETS, std::max<uint64_t>(64, want)));
+    const uint64_t runs = uint64_t(bb.n_buckets) * BUCKET_GROUPS;
+    bb.c_main_max = n * capacity_pct / 100 + runs * BUCKET_SLACK;
+    // the overflow run: 15% of the rows (and at least 64Ki)
+    bb.c_total = bb.c_main_max + n * 15 / 100 + 65536;
+    return bb;
+}
+
+void launch_bucket_plan(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
+                        int32_t n_zones, const uint32_t* stats, const BucketBuild& bb,
+                        uint32_t capacity_pct, uint32_t* hist, uint32_t* plan) {
+    const uint32_t m = uint32_t(std::min<uint64_t>(n, BUCKET_SAMPLE));
+    launch_timed(ctx, "bucket_plan", [&] {
+        hipLaunchKernelGGL(k_bucket_sample, dim3(256), dim3(TPB), 0, ctx->stream, recs, n, m,
+                           h_deg, n_zones, stats, bb.n_buckets, hist);
+        hipLaunchKernelGGL(k_bucket_plan, dim3(1), dim3(PLAN_TPB), 0, ctx->stream,
+                           (const uint32_t*)hist, bb.n_buckets, n, capacity_pct, plan);
+    });
+}
+
+void launch_make_keys_bucketed(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
+                               int32_t n_zones, PackLayout pl, const uint32_t* stats,
+                               const BucketBuild& bb, uint32_t* plan, xm_obj* part,
+                               uint64_t* words, int32_t* zminmax, uint32_t* bad, int lo_bit,
+                               int n_pass, uint32_t* digit_totals, const uint32_t* layout) {
+    if (n == 0) return;
+    if (n_pass < 0 || n_pass > SORT_MAX_PASSES || (n_pass > 0 && !digit_totals))
+        throw InvalidArg{"make_keys: bad digit histogram request"};
+    if (bb.c_total >= (uint64_t(1) << 32) || bb.n_buckets > MAX_BUCKETS)
+        throw InvalidArg{"bucketed build: too many rows"};
+    // 512 blocks (one per CU at a time: 152 KB of LDS each), rows in whole chunks
+    constexpr uint32_t BLOCKS = 512;
+    static_assert(BLOCKS % BUCKET_GROUPS == 0, "blocks cycle through the groups");
+    const uint64_t per = ((n + BLOCKS - 1) / BLOCKS + BK_CHUNK - 1) / BK_CHUNK * BK_CHUNK;
+    launch_timed(ctx, "make_keys", [&] {
+        hipLaunchKernelGGL(k_make_keys_bucketed, dim3(BLOCKS), dim3(BK_TPB), 0, ctx->stream, recs,
+                           uint32_t(n), uint32_t(per), h_deg, n_zones, pl, stats, bb.n_buckets,
+                           plan, uint32_t(bb.c_total), part, words, zminmax, bad, lo_bit, n_pass,
+                           digit_totals, layout);
     });
 }
 

@@ -76,6 +76,9 @@ struct xm_ctx {
     bool sort_fail_test = false;   // sorts fail by their look-back limit (error-path tests)
     bool fp32_rows = false;        // catalog builds also write the FP32 rows (gather pass)
     uint32_t chunk_rows = 0;       // A rows per pair-kernel chunk (0: CHUNK_ROWS)
+    uint64_t bucket_min_rows = uint64_t(1) << 26;  // one-segment builds of at least this
+                                   // many rows write their records into zone buckets
+    uint32_t bucket_capacity_pct = 115;  // bucket runs hold this % of the sampled estimate
     bool autotune = false;         // joins pick their shapes by measurement (tune table)
     // tune table: key (tune_key) -> measured shape, its time and the default's
     struct TuneEntry {
@@ -374,9 +377,46 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 // keys are distinct buffers (a row reads its left neighbour's word).  The
 // one-segment form takes the layout by value; the many-segment form reads it
 // from the segment table.
+// With xcd_runs the blocks are dealt so that each XCD takes one contiguous run of
+// rows (for records in zone buckets, whose reads then stay in that XCD's L2).
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
                    PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
-                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys);
+                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys, bool xcd_runs = false);
+
+// ---- one-segment build with the records in zone buckets
+// The key pass also moves every record into a bucket of consecutive zones (~2 MB of
+// records each), one run per bucket and XCD group, so that the gather, whose blocks
+// take one contiguous run of rows per XCD, reads each bucket from one L2 instead of
+// whole random 128-B lines.  The runs' sizes come from a sample of the records; a
+// record whose run is full goes to an overflow run; if that fills too, the plan has
+// failed (plan word BP_FAILED) and the caller rebuilds without buckets.
+constexpr uint32_t BUCKET_GROUPS = 8;       // bucket runs per bucket: one per XCD
+constexpr uint32_t MAX_BUCKETS = 2048;
+constexpr uint32_t BUCKET_SAMPLE = 1u << 20;
+constexpr uint32_t BUCKET_SLACK = 64;       // rows added to every run's estimate
+// largest bucketed catalog: run cursors (32-bit) stay below 2^32
+constexpr uint64_t BUCKET_MAX_ROWS = uint64_t(1) << 30;
+// plan words after the 2 x BUCKET_GROUPS x n_buckets run cursors and ends
+constexpr int BP_OVF_CURSOR = 0, BP_FAILED = 1, BP_WORDS = 4;
+struct BucketBuild {
+    uint32_t n_buckets;
+    uint64_t c_main_max;  // bound on the bucket runs' rows
+    uint64_t c_total;     // rows of the bucketed record array (runs + overflow)
+};
+BucketBuild bucket_build(uint64_t n, uint32_t capacity_pct);
+// plan: BUCKET_GROUPS x n_buckets cursors, as many run ends, BP_WORDS words; hist:
+// n_buckets counters (zero).  Samples the records' zones (bucket edges from the
+// layout statistics' zone range), sizes the runs and sets the cursors.
+void launch_bucket_plan(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
+                        int32_t n_zones, const uint32_t* stats, const BucketBuild& bb,
+                        uint32_t capacity_pct, uint32_t* hist, uint32_t* plan);
+// k_make_keys for bucketed records: the same words (input order, the row field =
+// the record's place in part), digit counts, zone range and bad count.
+void launch_make_keys_bucketed(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
+                               int32_t n_zones, PackLayout pl, const uint32_t* stats,
+                               const BucketBuild& bb, uint32_t* plan, xm_obj* part,
+                               uint64_t* words, int32_t* zminmax, uint32_t* bad, int lo_bit,
+                               int n_pass, uint32_t* digit_totals, const uint32_t* layout);
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
                        uint64_t n, Row64* rows, Row32* rows32, uint32_t* zoff, uint64_t n_zoff,
                        uint64_t* keys);
