@@ -558,14 +558,17 @@ def test_fp32_rows_and_fp64_rows_rounded_in_registers_agree(gpu_ctx):
     assert st_cvt["max_dist_err_arcsec"] == st_rows["max_dist_err_arcsec"] > 0
 
 
-@pytest.mark.parametrize("pct", [115, 70, 10])
-def test_bucketed_build_equals_plain(gpu_ctx, pct):
+@pytest.mark.parametrize("pct,n", [(115, 300_000), (70, 300_000), (10, 300_000),
+                                   (115, 5_000_123), (70, 5_000_123)])
+def test_bucketed_build_equals_plain(gpu_ctx, pct, n):
     # the key pass that moves the records into zone buckets, forced at this size:
     # with runs of 115% of the sampled estimate (the default), 70% (records spill
     # into the overflow run) and 10% (the overflow run fills too and the build falls
     # back to the plain one) the catalog is the plain build's row for row, and so
-    # are the joins
-    A, B, r = synth.config_small_patch(seed=71, n_a=300_000, n_b=300_000)
+    # are the joins.  At 300,000 rows each of the pass's 512 blocks takes at most
+    # one 4096-record chunk; at 5,000,123 rows a block takes three, the last block
+    # a partial one, so the pipelined chunks are all exercised.
+    A, B, r = synth.config_small_patch(seed=71, n_a=n, n_b=n)
     cp, cbp = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
     ep = cp.export()
     res = gpu_ctx.match(cp, cbp, r)

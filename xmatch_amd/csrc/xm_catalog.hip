@@ -546,21 +546,32 @@ inline dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + TPB - 1) / TPB)); }
 // then the digit counters (zero).
 // ---- one segment, records in zone buckets
 // Bucket of zone z: the sampled zone range [zlo, zhi] (layout statistics) cut into
-// n_buckets equal parts; zones outside it go to the first or last bucket.
-__device__ inline void bucket_range(const uint32_t* __restrict__ stats, int32_t n_zones,
-                                    int32_t* zlo, int64_t* span) {
+// n_buckets equal parts; zones outside it go to the first or last bucket.  The
+// scale nb / span is one FP64 product per record (an int64 division took ~80
+// instructions); the sample and the key pass use the same function.
+struct BucketMap {
+    int32_t zlo;
+    uint32_t nb;
+    double scale;
+};
+
+__device__ inline BucketMap bucket_map(const uint32_t* __restrict__ stats, int32_t n_zones,
+                                       uint32_t nb) {
     int32_t a = int32_t(stats[512]), b = int32_t(stats[513]);
     if (b < a) {  // nothing valid sampled
         a = 0;
         b = n_zones - 1;
     }
-    *zlo = a;
-    *span = int64_t(b) - int64_t(a) + 1;
+    BucketMap m;
+    m.zlo = a;
+    m.nb = nb;
+    m.scale = double(nb) / (double(b) - double(a) + 1.0);
+    return m;
 }
 
-__device__ inline uint32_t bucket_of(int32_t z, int32_t zlo, int64_t span, uint32_t nb) {
-    const int64_t k = (int64_t(z) - int64_t(zlo)) * int64_t(nb) / span;
-    return uint32_t(k < 0 ? 0 : (k >= int64_t(nb) ? int64_t(nb) - 1 : k));
+__device__ inline uint32_t bucket_of(int32_t z, const BucketMap& m) {
+    const double k = (double(z) - double(m.zlo)) * m.scale;
+    return k < 0.0 ? 0u : (k >= double(m.nb) ? m.nb - 1 : uint32_t(k));
 }
 
 // hist[b] += sampled records of bucket b (m records at an even stride)
@@ -571,15 +582,13 @@ __global__ __launch_bounds__(TPB) void k_bucket_sample(const xm_obj* __restrict_
     __shared__ uint32_t h[MAX_BUCKETS];
     for (uint32_t b = threadIdx.x; b < nb; b += TPB) h[b] = 0;
     __syncthreads();
-    int32_t zlo;
-    int64_t span;
-    bucket_range(stats, n_zones, &zlo, &span);
+    const BucketMap bm = bucket_map(stats, n_zones, nb);
     for (uint32_t t = blockIdx.x * TPB + threadIdx.x; t < m; t += gridDim.x * TPB) {
         const uint64_t i = uint64_t(t) * n / m;
         const double ra = recs[i].ra_deg;
         const double dec = recs[i].dec_deg;
         if (!valid_record(ra, dec)) continue;
-        atomicAdd(&h[bucket_of(zone_of(dec, h_deg, n_zones), zlo, span, nb)], 1u);
+        atomicAdd(&h[bucket_of(zone_of(dec, h_deg, n_zones), bm)], 1u);
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < nb; b += TPB)
@@ -654,11 +663,21 @@ __global__ __launch_bounds__(PLAN_TPB) void k_bucket_plan(const uint32_t* __rest
 // the records are regrouped by bucket in LDS and written as contiguous runs.  The
 // words go out in input order with the row field = the record's place in part, so
 // the sort orders ties by input row as the plain build does, and the catalog is the
-// same.
+// same.  A block holds one chunk's records in LDS, so a CU runs one block: the
+// next chunk's records are loaded into registers while this chunk reserves and
+// writes, and the staged records sit in one LDS array per field (8-byte lanes
+// read back without bank conflicts).  The 200M-row plane catalog (2048 buckets):
+// 3.47 ms per pass with records loaded where they are ranked, one 24-byte LDS
+// array and an int64 division per bucket; 2.97 ms with the prefetch, the LDS
+// arrays per field and the FP64 bucket map; 2.74 ms with each lane counting its
+// own digits.  Deeper pipelines measured slower: counting and reserving chunk k+1
+// while chunk k is written out (3.45 ms at 1024 threads, which spills registers;
+// 3.57 ms at 512 threads).
 constexpr uint32_t BK_TPB = 1024;
 constexpr uint32_t BK_CHUNK = 4096;
 constexpr uint32_t BK_ITEMS = BK_CHUNK / BK_TPB;
 constexpr uint32_t BK_NONE = 0xffffffffu;
+static_assert(MAX_BUCKETS <= 2 * BK_TPB, "a thread owns at most two buckets");
 __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
     const xm_obj* __restrict__ recs, uint32_t n, uint32_t per_blk, double h_deg,
     int32_t n_zones, PackLayout pl, const uint32_t* __restrict__ stats, uint32_t nb,
@@ -666,7 +685,8 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
     uint64_t* __restrict__ words, int32_t* __restrict__ zminmax, uint32_t* __restrict__ bad,
     int lo_bit, int n_pass, uint32_t* __restrict__ digit_totals,
     const uint32_t* __restrict__ layout) {
-    __shared__ xm_obj s_rec[BK_CHUNK];
+    __shared__ int64_t s_id[BK_CHUNK];
+    __shared__ double s_ra[BK_CHUNK], s_dec[BK_CHUNK];
     __shared__ uint32_t s_pos[BK_CHUNK];
     __shared__ uint32_t s_lb[MAX_BUCKETS + 1];  // counts, then chunk-local starts
     __shared__ uint32_t s_base[MAX_BUCKETS];    // the run start each bucket reserved
@@ -688,9 +708,8 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
     }
     const int bins = n_pass << db;
     for (int i = threadIdx.x; i < bins; i += BK_TPB) s_h[i] = 0;
-    int32_t zlo;
-    int64_t span;
-    bucket_range(stats, n_zones, &zlo, &span);
+    for (uint32_t b = threadIdx.x; b <= nb; b += BK_TPB) s_lb[b] = 0;
+    const BucketMap bm = bucket_map(stats, n_zones, nb);
     const uint32_t runs = nb * BUCKET_GROUPS;
     const uint32_t g = blockIdx.x % BUCKET_GROUPS;
     uint32_t* const cursor = plan + g * nb;
@@ -701,11 +720,15 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
     uint32_t nbad = 0;
     const uint64_t lo = uint64_t(blockIdx.x) * per_blk;
     const uint64_t hi = lo + per_blk < n ? lo + per_blk : n;
+    xm_obj x[BK_ITEMS];
+#pragma unroll
+    for (uint32_t j = 0; j < BK_ITEMS; j++) {
+        const uint64_t i = lo + j * BK_TPB + threadIdx.x;
+        if (i < hi) x[j] = recs[i];
+    }
+    __syncthreads();
     for (uint64_t c0 = lo; c0 < hi; c0 += BK_CHUNK) {
         const uint32_t m = uint32_t(hi - c0 < BK_CHUNK ? hi - c0 : BK_CHUNK);
-        for (uint32_t b = threadIdx.x; b <= nb; b += BK_TPB) s_lb[b] = 0;
-        __syncthreads();
-        xm_obj x[BK_ITEMS];
         uint64_t wd[BK_ITEMS];
         uint32_t bk[BK_ITEMS], rk[BK_ITEMS];
 #pragma unroll
@@ -715,21 +738,34 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
             wd[j] = ~uint64_t(0);
             bool valid = false;
             if (sl < m) {
-                x[j] = recs[c0 + sl];
                 valid = valid_record(x[j].ra_deg, x[j].dec_deg);
                 if (valid) {
                     const int32_t z = zone_of(x[j].dec_deg, h_deg, n_zones);
                     wd[j] = pack_word(pl, z, x[j].ra_deg, 0);
                     zmn = min(zmn, z);
                     zmx = max(zmx, z);
-                    bk[j] = bucket_of(z, zlo, span, nb);
+                    bk[j] = bucket_of(z, bm);
                     rk[j] = atomicAdd(&s_lb[bk[j]], 1u);
                 } else {
                     nbad++;
                 }
             }
-            if (db == 9) count_digits<9>(s_h, wd[j], valid, lo_bit, n_pass);
-            else count_digits<8>(s_h, wd[j], valid, lo_bit, n_pass);
+            // records in random order seldom share a digit across a wavefront, so
+            // each counts its own (no uniform-digit test as in count_digits)
+            if (valid) {
+                const uint32_t dm = (1u << db) - 1u;
+                for (int p = 0; p < n_pass; p++)
+                    atomicAdd(&s_h[(uint32_t(p) << db) + (uint32_t(wd[j] >> (lo_bit + db * p)) & dm)],
+                              1u);
+            }
+        }
+        // the next chunk's records, in flight while this one reserves and writes
+        const uint64_t c1 = c0 + BK_CHUNK;
+        xm_obj xn[BK_ITEMS];
+#pragma unroll
+        for (uint32_t j = 0; j < BK_ITEMS; j++) {
+            const uint64_t i = c1 + j * BK_TPB + threadIdx.x;
+            if (i < hi) xn[j] = recs[i];
         }
         __syncthreads();
         // exclusive scan of the bucket counts (two per thread), s_lb[nb] = total
@@ -758,22 +794,33 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
             }
         }
         __syncthreads();
-        // each bucket reserves its rows in its run, the rest in the overflow run
-        for (uint32_t b = threadIdx.x; b < nb; b += BK_TPB) {
-            const uint32_t cnt = s_lb[b + 1] - s_lb[b];
-            if (!cnt) continue;
-            const uint32_t base = atomicAdd(&cursor[b], cnt);
-            const uint32_t e = run_end[b];
-            const uint32_t fit = base >= e ? 0u : (e - base < cnt ? e - base : cnt);
-            s_base[b] = base;
-            s_fit[b] = fit;
-            if (fit < cnt) {
-                const uint32_t ob = atomicAdd(&ovf[BP_OVF_CURSOR], cnt - fit);
-                if (uint64_t(ob) + (cnt - fit) > ovf_end) {
-                    atomicOr(&ovf[BP_FAILED], 1u);
-                    s_obase[b] = BK_NONE;
-                } else {
-                    s_obase[b] = ob;
+        // each bucket reserves its rows in its run, the rest in the overflow run;
+        // a thread's two buckets issue their atomics together
+        {
+            uint32_t cnt[2], base[2], e[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint32_t b = threadIdx.x + q * BK_TPB;
+                cnt[q] = b < nb ? s_lb[b + 1] - s_lb[b] : 0u;
+                base[q] = cnt[q] ? atomicAdd(&cursor[b], cnt[q]) : 0u;
+                e[q] = cnt[q] ? run_end[b] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint32_t b = threadIdx.x + q * BK_TPB;
+                if (!cnt[q]) continue;
+                const uint32_t fit =
+                    base[q] >= e[q] ? 0u : (e[q] - base[q] < cnt[q] ? e[q] - base[q] : cnt[q]);
+                s_base[b] = base[q];
+                s_fit[b] = fit;
+                if (fit < cnt[q]) {
+                    const uint32_t ob = atomicAdd(&ovf[BP_OVF_CURSOR], cnt[q] - fit);
+                    if (uint64_t(ob) + (cnt[q] - fit) > ovf_end) {
+                        atomicOr(&ovf[BP_FAILED], 1u);
+                        s_obase[b] = BK_NONE;
+                    } else {
+                        s_obase[b] = ob;
+                    }
                 }
             }
         }
@@ -788,18 +835,30 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
                 if (rk[j] < s_fit[b]) pos = s_base[b] + rk[j];
                 else if (s_obase[b] != BK_NONE) pos = s_obase[b] + (rk[j] - s_fit[b]);
                 const uint32_t slot = s_lb[b] + rk[j];
-                s_rec[slot] = x[j];
+                s_id[slot] = x[j].id;
+                s_ra[slot] = x[j].ra_deg;
+                s_dec[slot] = x[j].dec_deg;
                 s_pos[slot] = pos;
             }
             words[c0 + sl] = pos != BK_NONE ? (wd[j] | pos) : ~uint64_t(0);
         }
-        __syncthreads();
         const uint32_t staged = s_lb[nb];
+        __syncthreads();
         for (uint32_t sl = threadIdx.x; sl < staged; sl += BK_TPB) {
             const uint32_t pos = s_pos[sl];
-            if (pos != BK_NONE) part[pos] = s_rec[sl];
+            if (pos != BK_NONE) {
+                xm_obj r;
+                r.id = s_id[sl];
+                r.ra_deg = s_ra[sl];
+                r.dec_deg = s_dec[sl];
+                part[pos] = r;
+            }
         }
+        // the counts start from zero for the next chunk (nothing reads them now)
+        for (uint32_t b = threadIdx.x; b <= nb; b += BK_TPB) s_lb[b] = 0;
         __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < BK_ITEMS; j++) x[j] = xn[j];
     }
     // zone range and bad count: one atomic each per block
 #pragma unroll
@@ -964,8 +1023,12 @@ void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v) {
 
 BucketBuild bucket_build(uint64_t n, uint32_t capacity_pct) {
     BucketBuild bb;
-    // ~2 MB of records a bucket, in [64, MAX_BUCKETS]
-    const uint64_t want = n * sizeof(xm_obj) / (uint64_t(2) << 20);
+    // ~4 MB of records a bucket, in [64, MAX_BUCKETS].  Fewer buckets make the key
+    // pass cheaper (fewer reservations, longer runs) and the gather dearer (a bucket
+    // outgrows the XCD's 4 MB L2).  The 200M-row plane catalog, key pass + gather per
+    // catalog: 2 MB (2048 buckets) 2.74 + 3.88 ms, 3 MB 2.60 + 3.98, 4 MB 2.49 +
+    // 4.07, 6 MB 2.39 + 4.20, 8 MB 2.36 + 4.34.
+    const uint64_t want = n * sizeof(xm_obj) / (uint64_t(4) << 20);
     bb.n_buckets = uint32_t(std::min<uint64_t>(MAX_BUCKETS, std::max<uint64_t>(64, want)));
     const uint64_t runs = uint64_t(bb.n_buckets) * BUCKET_GROUPS;
     bb.c_main_max = n * capacity_pct / 100 + runs * BUCKET_SLACK;

@@ -384,7 +384,7 @@ void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint6
                    uint32_t* zoff, uint64_t n_zoff, uint64_t* keys, bool xcd_runs = false);
 
 // ---- one-segment build with the records in zone buckets
-// The key pass also moves every record into a bucket of consecutive zones (~2 MB of
+// The key pass also moves every record into a bucket of consecutive zones (~4 MB of
 // records each), one run per bucket and XCD group, so that the gather, whose blocks
 // take one contiguous run of rows per XCD, reads each bucket from one L2 instead of
 // whole random 128-B lines.  The runs' sizes come from a sample of the records; a
