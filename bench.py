@@ -55,7 +55,7 @@ BYTES_PER_SOURCE = {
     "radix_scatter": 16.0,   # read packed word 8, write packed word 8
     "radix_hist": 8.0,       # read packed word (once per sort, every digit)
     "make_keys": 32.0,       # read 24-B record, write packed word 8
-    # catalogs of 2^26 rows and more: the key pass also writes each record into its
+    # catalogs of 2^24 rows and more: the key pass also writes each record into its
     # zone bucket: read 24, write 24 + 8
     "make_keys_bucketed": 56.0,
     "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
@@ -69,7 +69,7 @@ KERNELS = ["layout", "bucket_plan", "make_keys", "radix_hist", "radix_offsets",
 PROFILE_NAMES = {"pair_fp32": "pair<Fp32Cvt>", "pair_fp64": "pair<Fp64>"}
 # the library's default bucket_min_rows: one-segment catalogs from this size up are
 # built with their records in zone buckets
-BUCKET_MIN_ROWS = 1 << 26
+BUCKET_MIN_ROWS = 1 << 24
 
 PLANE_SEED = 2003  # device generator key of the headline catalogs
 # the forms of the headline join: (name, precision, fp32_rows, --mode key).  The FP32

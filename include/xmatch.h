@@ -101,8 +101,9 @@ xm_status xm_ctx_kernel_time(xm_ctx *ctx, const char *name, double *total_ms, ui
 xm_status xm_ctx_reset_timing(xm_ctx *ctx);
 /* Tuning knobs of the context (no counterpart in the reference, whose kernel
  * shape was fixed at compile time, Worker.cu:22,351-353):
- *   "dense_min"  an A chunk (up to 256 rows of one zone) is joined densely
- *                (every B row of its window against every chunk row) when a B row
+ *   "dense_min"  an A chunk (up to chunk_rows rows of one zone, 512 by default) is
+ *                joined densely (every B row of its window against every chunk row)
+ *                when a B row
  *                is expected to meet at least this many chunk rows, else sparsely
  *                (each B row against the chunk rows within the ra window).
  *                0 = always dense.  Default 16.
@@ -118,6 +119,13 @@ xm_status xm_ctx_reset_timing(xm_ctx *ctx);
  *                (XM_FP32_FILTER_FP64_RECHECK) of two catalogs that carry them reads
  *                them; otherwise it rounds the FP64 rows to FP32 in registers, the
  *                same values, and reads no more than the FP64 form.  Default 0.
+ *   "bucket_min_rows" one-segment catalogs of at least this many rows are built
+ *                with their records moved into zone buckets by the key pass, so that
+ *                the gather reads them from one XCD's L2.  0 = never.  Default 2^24.
+ *   "bucket_capacity_pct" a bucket's runs hold this % (1..1000) of its sampled
+ *                estimate; records past a full run go to an overflow run, and a
+ *                build whose overflow run fills is redone without buckets.
+ *                Default 115.
  *   "sort_fail_test" 1 = every catalog sort of more than one tile (8192 rows)
  *                fails through its look-back's bounded wait, and catalog creation
  *                returns XM_EINTERNAL: for tests of that error path.  Default 0.

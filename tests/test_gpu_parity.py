@@ -569,14 +569,16 @@ def test_bucketed_build_equals_plain(gpu_ctx, pct, n):
     # one 4096-record chunk; at 5,000,123 rows a block takes three, the last block
     # a partial one, so the pipelined chunks are all exercised.
     A, B, r = synth.config_small_patch(seed=71, n_a=n, n_b=n)
-    cp, cbp = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
-    ep = cp.export()
-    res = gpu_ctx.match(cp, cbp, r)
-    ref = res.to_host()
-    res.close(); cp.close(); cbp.close()
-    gpu_ctx.set_param("bucket_min_rows", 1)
-    gpu_ctx.set_param("bucket_capacity_pct", pct)
+    default_min = gpu_ctx.get_param("bucket_min_rows")
     try:
+        gpu_ctx.set_param("bucket_min_rows", 0)   # the plain build
+        cp, cbp = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
+        ep = cp.export()
+        res = gpu_ctx.match(cp, cbp, r)
+        ref = res.to_host()
+        res.close(); cp.close(); cbp.close()
+        gpu_ctx.set_param("bucket_min_rows", 1)
+        gpu_ctx.set_param("bucket_capacity_pct", pct)
         assert gpu_ctx.get_param("bucket_capacity_pct") == pct
         ca, cb = gpu_ctx.catalog(A, r), gpu_ctx.catalog(B, r)
         eb = ca.export()
@@ -584,7 +586,7 @@ def test_bucketed_build_equals_plain(gpu_ctx, pct, n):
         got = res.to_host()
         res.close(); ca.close(); cb.close()
     finally:
-        gpu_ctx.set_param("bucket_min_rows", 2**26)
+        gpu_ctx.set_param("bucket_min_rows", default_min)
         gpu_ctx.set_param("bucket_capacity_pct", 115)
     for f in ep:
         assert np.array_equal(ep[f], eb[f]), f

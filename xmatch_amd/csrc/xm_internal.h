@@ -76,7 +76,7 @@ struct xm_ctx {
     bool sort_fail_test = false;   // sorts fail by their look-back limit (error-path tests)
     bool fp32_rows = false;        // catalog builds also write the FP32 rows (gather pass)
     uint32_t chunk_rows = 0;       // A rows per pair-kernel chunk (0: CHUNK_ROWS)
-    uint64_t bucket_min_rows = uint64_t(1) << 26;  // one-segment builds of at least this
+    uint64_t bucket_min_rows = uint64_t(1) << 24;  // one-segment builds of at least this
                                    // many rows write their records into zone buckets
     uint32_t bucket_capacity_pct = 115;  // bucket runs hold this % of the sampled estimate
     bool autotune = false;         // joins pick their shapes by measurement (tune table)
