@@ -397,26 +397,38 @@ def test_invalid_records_rejected(gpu_ctx, bad):
     assert len(got[0]) == 1
 
 
-def test_failed_builds_return_their_memory(gpu_ctx):
+@pytest.mark.parametrize("bucketed", [False, True])
+def test_failed_builds_return_their_memory(gpu_ctx, bucketed):
     # A build that fails (bad records; a sort whose look-back fails) returns every
     # device block it took: free memory, with the context's cached blocks counted
-    # as free, is the same before and after.
+    # as free, is the same before and after.  Also with the records moved into
+    # zone buckets (forced at this size), whose build takes the bucket array too.
     A, _, r = synth.config_small_patch(seed=5, n_a=2_000_000, n_b=10)
-    gpu_ctx.catalog(A, r).close()   # the context's blocks and error word exist
-    gpu_ctx.synchronize()
-    free0 = gpu_ctx.mem_info()[0]
-    bad = A.copy()
-    bad["dec"][1_000_000] = np.nan
-    with pytest.raises(xm.XmatchError):
-        gpu_ctx.catalog(bad, r)
+    default_min = gpu_ctx.get_param("bucket_min_rows")
     try:
-        gpu_ctx.set_param("sort_fail_test", 1)
-        with pytest.raises(xm.XmatchError):
-            gpu_ctx.catalog(A, r)
+        gpu_ctx.set_param("bucket_min_rows", 1 if bucketed else 0)
+        gpu_ctx.catalog(A, r).close()   # the context's blocks and error word exist
+        gpu_ctx.synchronize()
+        free0 = gpu_ctx.mem_info()[0]
+        bad = A.copy()
+        bad["dec"][1_000_000] = np.nan
+        with pytest.raises(xm.XmatchError) as ei:
+            gpu_ctx.catalog(bad, r)
+        assert ei.value.status == xm.XM_EINVAL
+        try:
+            gpu_ctx.set_param("sort_fail_test", 1)
+            with pytest.raises(xm.XmatchError):
+                gpu_ctx.catalog(A, r)
+        finally:
+            gpu_ctx.set_param("sort_fail_test", 0)
+        gpu_ctx.synchronize()
+        free1 = gpu_ctx.mem_info()[0]
+        # the catalog builds again, as before the failures
+        ca = gpu_ctx.catalog(A, r)
+        assert ca.info()["n"] == len(A)
+        ca.close()
     finally:
-        gpu_ctx.set_param("sort_fail_test", 0)
-    gpu_ctx.synchronize()
-    free1 = gpu_ctx.mem_info()[0]
+        gpu_ctx.set_param("bucket_min_rows", default_min)
     # a leak would hold at least the 16-MB key buffers of the 2M-row sort
     assert abs(int(free1) - int(free0)) < (4 << 20)
 
