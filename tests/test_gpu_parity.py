@@ -605,6 +605,52 @@ def test_bucketed_build_equals_plain(gpu_ctx, pct, n):
     assert_same(got, ref)
 
 
+@pytest.mark.parametrize("n,r_arcsec,cap_deg", [(1_700_000, 0.5, 0.1), (2_000_000, 4.0, 1.0)])
+def test_bucketed_build_clustered_equals_plain(gpu_ctx, n, r_arcsec, cap_deg):
+    # zone buckets come from a sample of the records; here 90% of them sit in a
+    # small cap (nearly every record lands in one bucket) and 10% are spread over
+    # the sky (zones the layout sample may miss fall into the first or last
+    # bucket).  The catalog, forced bucketed, equals the plain build row for row,
+    # and so does the self-join.  At 1.7M rows and 0.5" the bucket array would
+    # need a wider row field than the input, costing the key an ra bit, so it is
+    # cut to the input's field (its overflow run shrinks); at 2M rows and 4" the
+    # key keeps all 24 ra bits either way.
+    rng = np.random.default_rng(zlib.crc32(b"clustered %d" % n))
+    n_c = n * 9 // 10
+    ra_c, dec_c = synth.uniform_cap(rng, n_c, 123.4, -45.6, cap_deg)
+    ra_s, dec_s = synth.uniform_sphere(rng, n - n_c)
+    A = synth.make_objs(np.arange(n), np.concatenate([ra_c, ra_s]),
+                        np.concatenate([dec_c, dec_s]))
+    A = A[rng.permutation(n)]
+    r = r_arcsec / 3600
+    default_min = gpu_ctx.get_param("bucket_min_rows")
+    try:
+        gpu_ctx.set_param("bucket_min_rows", 0)
+        cp = gpu_ctx.catalog(A, r)
+        ep = cp.export()
+        res = gpu_ctx.match(cp, cp, r)
+        ref = res.to_host()
+        res.close(); cp.close()
+        gpu_ctx.set_param("bucket_min_rows", 1)
+        gpu_ctx.reset_timing()
+        gpu_ctx.set_timing(True, only="bucket_plan")
+        ca = gpu_ctx.catalog(A, r)
+        gpu_ctx.set_timing(False)
+        _, planned = gpu_ctx.kernel_time("bucket_plan")
+        gpu_ctx.reset_timing()
+        eb = ca.export()
+        res = gpu_ctx.match(ca, ca, r)
+        got = res.to_host()
+        res.close(); ca.close()
+    finally:
+        gpu_ctx.set_param("bucket_min_rows", default_min)
+    assert planned == 1   # the build took the bucketed path (its plan was made)
+    for f in ep:
+        assert np.array_equal(ep[f], eb[f]), f
+    assert_same(got, ref)
+    assert len(ref[0]) >= n   # every source matches itself
+
+
 def test_bucket_knobs_reject_bad_values(gpu_ctx):
     for name, v in (("bucket_min_rows", -1), ("bucket_capacity_pct", 0),
                     ("bucket_capacity_pct", 1001)):

@@ -348,14 +348,32 @@ void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
 // absolute zone (the zone range comes back with the same pass).  A catalog of at
 // least bucket_min_rows rows has that pass also move its records into zone buckets
 // (BucketBuild), which the gather then reads from one XCD's L2 each; returns false,
-// having built nothing, when the buckets' sampled plan failed.
+// having built nothing, when the buckets' sampled plan failed or the bucket array
+// does not fit the row field.
 bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
     SegDesc& d = cat->seg[0];
     const uint64_t n = d.n;
-    const BucketBuild bb = bucket_build(n, ctx->bucket_capacity_pct);
+    BucketBuild bb = bucket_build(n, ctx->bucket_capacity_pct);
     // the row field holds the input row, or the record's place among the buckets
-    PackLayout pl = pack_layout(bucketed ? bb.c_total : n, d.n_zones);
+    PackLayout pl = pack_layout(n, d.n_zones);
     if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
+    if (bucketed) {
+        // The bucketed words keep the plain build's zone and ra bits, so the sort
+        // orders the rows identically (ties by input row) and the catalog is the
+        // same.  Where the wider row field the bucket array needs would cost an ra
+        // bit, the array is cut to the positions the input's row field addresses,
+        // the overflow run taking what is left; with less than 2% of the rows left
+        // for it the build is plain.
+        PackLayout pb = pack_layout(bb.c_total, d.n_zones);
+        if (pb.ra_bits > ctx->key_ra_bits) pb.ra_bits = ctx->key_ra_bits;
+        if (pb.ra_bits == pl.ra_bits) {
+            pl = pb;
+        } else {
+            const uint64_t room = uint64_t(1) << pl.idx_bits;
+            if (room < bb.c_main_max + n / 50) return false;
+            bb.c_total = room;
+        }
+    }
     d.idx_bits = pl.idx_bits;
     d.zone_bits = pl.zone_bits;
     d.zone_base = 0;
