@@ -391,10 +391,21 @@ __device__ inline void store_row32_nt(Row32* p, const Row64& r) {
     __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
 }
 
+// Blocks are dealt to the 8 XCDs round-robin.  The block that launch order b
+// stands for is remapped so that XCD x takes the x-th contiguous run of rows (the
+// blocks past the last whole round keep their place): the rows an XCD gathers at
+// once draw their records from a narrow range of the input (one zone bucket of a
+// bucketed build, a few segments of a many-segment one) held in its own L2.
+__device__ inline uint32_t xcd_run_block(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb / 8;
+    return b < per * 8 ? (b % 8) * per + b / 8 : b;
+}
+
 // k_gather takes one row per thread in blocks of GATHER_TPB: 50M random-order
 // records, 256 threads 1.61 ms, 512 1.60 ms, 1024 1.59 ms.  k_gather_seg keeps
 // blocks of TPB: the 10,000-segment batch's gather took 49.5 ms with 256 threads and
-// 50.1 ms with 1024.
+// 50.1 ms with 1024; dealt in runs per XCD, 46.0 ms (46.1 with 512 threads, 46.7
+// with 1024).
 constexpr int GATHER_TPB = 1024;
 
 __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ seg, int seg_shift,
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
                                                     uint32_t* __restrict__ zoff, uint32_t n_zoff,
                                                     uint64_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ sort_err) {
-    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    const uint32_t i = xcd_run_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= n || *sort_err) return;
     const uint64_t wd = words[i];
     const uint32_t s = seg_shift < 64 ? uint32_t(wd >> seg_shift) : 0u;
@@ -447,12 +458,7 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
                                                 uint64_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ sort_err,
                                                 uint32_t xcd_runs) {
-    uint32_t blk = blockIdx.x;
-    if (xcd_runs) {
-        // blocks are dealt to the XCDs round-robin: XCD x takes the x-th run of rows
-        const uint32_t per = gridDim.x / 8;
-        if (blk < per * 8) blk = (blk % 8) * per + blk / 8;
-    }
+    const uint32_t blk = xcd_runs ? xcd_run_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t i = blk * GATHER_TPB + threadIdx.x;
     if (i >= n || *sort_err) return;
     const uint64_t wd = words[i];
