@@ -696,7 +696,11 @@ int run(Options& o) {
         uint64_t band = o.band_rows ? o.band_rows : std::max<uint64_t>(free_b / 3 / 100, 1);
         if (o.num_obj) band = std::min(band, o.num_obj);
         plan = plan_bands(fa.fd, nA, zh_deg, n_zones, std::min(band, MAX_ROWS));
-        plan.round_rows = o.round_rows ? o.round_rows : std::max<uint64_t>(free_b / 3 / 200, 1);
+        // a round never holds more than file B (its two page-locked buffers would
+        // otherwise be sized for what the device could hold: 2 x 12 GB on one MI355X)
+        plan.round_rows = o.round_rows ? o.round_rows
+                                       : std::min(std::max<uint64_t>(free_b / 3 / 200, 1),
+                                                  std::max<uint64_t>(nB, 1));
         plan.round_rows = std::min(plan.round_rows, MAX_ROWS);
         if (!manifest.empty()) write_manifest(manifest, manifest_text(identity, plan));
     }
@@ -722,21 +726,33 @@ int run(Options& o) {
 
     uint64_t total = 0, skipped = 0, k = 0;
     for (size_t band = 0; band < plan.lo.size(); band++) {
+        // B's rows of the band's zones and nz + 2 zones either side: the first round
+        // is read while A's band is read, uploaded and prepared
+        ZoneReader rb(fb.fd, nB, zh_deg, n_zones, plan.lo[band] - nz - 2, plan.hi[band] + nz + 2);
+        uint64_t got[2] = {0, 0};
+        bool first_ok = true;
+        ThreadOwner first;
+        first.t = std::thread([&] { first_ok = rb.next(hb[0], plan.round_rows, &got[0]); });
         // A's rows of the band, resident while B streams past
         ZoneReader ra(fa.fd, nA, zh_deg, n_zones, plan.lo[band], plan.hi[band]);
         uint64_t na = 0;
         if (!ra.next(ha.p, plan.a_rows[band], &na) || na != plan.a_rows[band])
             throw CliError{3, "read error on " + o.inputs[0] + " (or it changed since the plan)"};
+        const double t_read_a = since(t_start);
         CatalogOwner A{prepare(C, ha.p, na, o.num_obj, zh_deg, "prepare A")};
         uint64_t nsegA = 0;
         xm_catalog_segment_count(A.c, &nsegA);
-        // B's rows of the band's zones and nz + 2 zones either side
-        ZoneReader rb(fb.fd, nB, zh_deg, n_zones, plan.lo[band] - nz - 2, plan.hi[band] + nz + 2);
-        uint64_t got[2] = {0, 0};
-        if (!rb.next(hb[0], plan.round_rows, &got[0])) throw CliError{3, "read error on " + o.inputs[1]};
-        if (o.verbosity)
+        const double t_prep_a = since(t_start);
+        first.t.join();
+        if (!first_ok) throw CliError{3, "read error on " + o.inputs[1]};
+        if (o.verbosity) {
             std::cerr << "band " << band << ": zones " << plan.lo[band] << ".." << plan.hi[band]
-                      << ", " << na << " A records" << std::endl;
+                      << ", " << na << " A records";
+            if (o.verbosity >= 2)
+                std::cerr << " (s from start: read " << t_read_a << ", prepared " << t_prep_a
+                          << "; first B round read by " << since(t_start) << ")";
+            std::cerr << std::endl;
+        }
         for (int cur = 0; got[cur] > 0; cur ^= 1) {
             // the next round is read while this one joins
             bool next_ok = true;
@@ -770,10 +786,13 @@ int run(Options& o) {
                 total += m;
                 if (!o.outdir.empty() && !write_round(C, res.r, m, path, o.format))
                     throw CliError{4, "cannot write " + path};
-                if (o.verbosity)
+                if (o.verbosity) {
                     std::cerr << "round " << k << ": band " << band << ", " << n << " B records, "
                               << jobs.size() << " jobs, " << m << " matches, join " << st.join_ms
-                              << " ms" << std::endl;
+                              << " ms";
+                    if (o.verbosity >= 2) std::cerr << " (done " << since(t_start) << " s from start)";
+                    std::cerr << std::endl;
+                }
             }
             k++;
             reader.t.join();
