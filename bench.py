@@ -683,13 +683,19 @@ def make_serving_queries(ctx, res_cat, res_host, n_queries, seed):
 def run_serve(args):
     """Serving form: a resident 5e8-source catalog (galactic-plane density, h = 1")
     answers requests arriving as a Poisson process; latency = submit -> done,
-    measured by the server; p50 / p99 per arrival rate."""
+    measured by the server; p50 / p99 per arrival rate.  --precision fp32: the
+    resident catalog stores FP32 unit vectors beside its FP64 rows (fp32_rows) and
+    the server runs the FP32 filter with the FP64 recheck."""
     import threading
     n_res = args.n or 500_000_000
     ctx = xm.Context(0)
+    fp32 = args.precision == "fp32"
+    prec = xm.FP32_FILTER_FP64_RECHECK if fp32 else xm.FP64_PLAIN
     plane = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
     t0 = time.time()
     d_res = ctx.synth_fresh(plane, n_res, 5000 + args.seed, stream=0, id0=0)
+    if fp32:
+        ctx.set_param("fp32_rows", 1)
     res_cat = ctx.catalog(d_res, 1.0 / 3600)
     res_host = ctx.download(d_res)
     d_res.free()
@@ -703,7 +709,7 @@ def run_serve(args):
         queries = make_serving_queries(ctx, res_cat, res_host, n_q, seed=int(rate * 1000) + args.seed)
         log("rate %.0f q/s: %d requests generated in %.1f s" % (rate, n_q, time.time() - t0))
         gaps = rng.exponential(1.0 / rate, n_q)
-        with xm.Server(ctx, res_cat) as srv:
+        with xm.Server(ctx, res_cat, precision=prec) as srv:
             # warm the server's buffers with a few requests
             for q, r in queries[:4]:
                 srv.result(srv.submit(q, r))
@@ -762,7 +768,9 @@ def run_serve(args):
                                   "h = 1 arcsec), Poisson arrivals, requests N ~ logU[1, 1e5] in "
                                   "caps of U[1, 30] arcmin, 70%% 0.5-arcsec copies of resident "
                                   "sources, r ~ U[1, 5] arcsec" % n_res,
-                      "resident_sources": n_res, "duration_s_per_rate": args.duration},
+                      "resident_sources": n_res, "duration_s_per_rate": args.duration,
+                      "precision": "fp32_filter_fp64_recheck, FP32 rows stored" if fp32
+                      else "fp64_plain"},
            "rates": per_rate}
     res_cat.close()
     ctx.close()
@@ -783,7 +791,8 @@ def main():
                     help="serve config: seconds of arrivals per rate")
     ap.add_argument("--jobs", type=int, default=10_000, help="jobs of the batch config")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
-                    help="batch config: FP64 plain or the FP32 filter with FP64 recheck")
+                    help="batch and serve configs: FP64 plain or the FP32 filter with FP64 "
+                         "recheck")
     ap.add_argument("--no-work-counts", action="store_true",
                     help="batch config: skip the host work counts")
     ap.add_argument("--gpus", type=int, default=1)
