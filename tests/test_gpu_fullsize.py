@@ -168,7 +168,9 @@ def _digest(ida, idb, dist):
 @pytest.fixture(scope="module")
 def plane(gpu_ctx):
     """The 200M x 200M galactic-plane join at 1 arcsec, records generated on the
-    device: both forms' pairs, and the records on the host."""
+    device: the pairs of FP64 plain, of the FP32 filter on the FP64 rows rounded in
+    registers and of the FP32 filter on FP32 rows the builds store, and the records
+    on the host."""
     import xmatch_amd as xm
     n, r = 200_000_000, 1.0 / 3600.0
     shp = xm.shape(xm.SHAPE_PLANE, synth.PLANE_POLE_RA, synth.PLANE_POLE_DEC, 20.0, 10.0)
@@ -182,6 +184,14 @@ def plane(gpu_ctx):
         out[name] = res.to_host(sort=False)
         res.close()
     ca.close(); cb.close()
+    try:
+        gpu_ctx.set_param("fp32_rows", 1)
+        ca, cb = gpu_ctx.catalog(dA, r), gpu_ctx.catalog(dB, r)
+    finally:
+        gpu_ctx.set_param("fp32_rows", 0)
+    res = gpu_ctx.match(ca, cb, r, precision=xm.FP32_FILTER_FP64_RECHECK)
+    out["fp32rows"] = res.to_host(sort=False)
+    res.close(); ca.close(); cb.close()
     A, B = gpu_ctx.download(dA), gpu_ctx.download(dB)
     dA.free(); dB.free()
     gpu_ctx.release_memory()
@@ -189,11 +199,13 @@ def plane(gpu_ctx):
 
 
 def test_full_plane_fp32_filter_equals_fp64(plane):
-    # the FP32 filter with FP64 recheck returns the FP64 set, distances bit for bit
+    # the FP32 filter with FP64 recheck returns the FP64 set, distances bit for bit,
+    # whether it rounds the FP64 rows in registers or reads stored FP32 rows
     _, _, _, _, out = plane
     dig = {k: _digest(*v) for k, v in out.items()}
     assert 99_500_000 < dig["fp64"][0] < 101_000_000
     assert dig["fp32"] == dig["fp64"]
+    assert dig["fp32rows"] == dig["fp64"]
 
 
 def test_full_plane_densest_band_vs_oracle(plane):
