@@ -325,12 +325,14 @@ __global__ __launch_bounds__(TPB) void k_seg_range(const SegDesc* __restrict__ s
 // words for the radix sort (segment bits are uniform over long runs, so
 // count_digits adds a wavefront's shared digit once) and adds its counts to
 // digit_totals at the end.
+template <int DB>
 __global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ seg, uint32_t n_seg,
                                                    uint64_t n, int seg_shift,
                                                    uint64_t* __restrict__ words, int lo_bit,
                                                    int n_pass, uint32_t* __restrict__ digit_totals) {
-    __shared__ uint32_t s_h[SORT_MAX_PASSES * 256];
-    for (int i = threadIdx.x; i < n_pass * 256; i += TPB) s_h[i] = 0;
+    constexpr int R = 1 << DB;
+    __shared__ uint32_t s_h[SORT_HIST_BINS];
+    for (int i = threadIdx.x; i < n_pass * R; i += TPB) s_h[i] = 0;
     __syncthreads();
     const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint64_t t0 = uint64_t(blockIdx.x) * per;
@@ -353,12 +355,12 @@ __global__ __launch_bounds__(TPB) void k_seg_words(const SegDesc* __restrict__ s
                 const uint64_t q = ra_quantum(ra) >> (RA_BITS - rb);
                 const uint64_t word = top | (uint64_t(uint32_t(z)) << (rb + ib)) | (q << ib) | (i - base);
                 words[i] = word;
-                count_digits<8>(s_h, word, true, lo_bit, n_pass);
+                count_digits<DB>(s_h, word, true, lo_bit, n_pass);
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n_pass * 256; i += TPB)
+    for (int i = threadIdx.x; i < n_pass * R; i += TPB)
         if (s_h[i]) atomicAdd(&digit_totals[i], s_h[i]);
 }
 
@@ -965,14 +967,21 @@ void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 }
 
 void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
-                      uint64_t* words, int lo_bit, int n_pass, uint32_t* digit_totals) {
+                      uint64_t* words, int lo_bit, int n_pass, uint32_t* digit_totals,
+                      int digit_bits) {
     if (n == 0 || n_seg == 0) return;
-    if (n_pass < 0 || n_pass > SORT_MAX_PASSES || (n_pass > 0 && !digit_totals))
+    if ((digit_bits != 8 && digit_bits != 9) || n_pass < 0 ||
+        n_pass > (digit_bits == 8 ? SORT_MAX_PASSES : SEG9_MAX_PASSES) ||
+        (n_pass > 0 && !digit_totals))
         throw InvalidArg{"seg_words: bad digit histogram request"};
     const uint32_t nblk = uint32_t(std::min<uint64_t>((n + TPB - 1) / TPB, MAKE_KEYS_BLOCKS));
     launch_timed(ctx, "make_keys", [&] {
-        hipLaunchKernelGGL(k_seg_words, dim3(nblk), dim3(TPB), 0, ctx->stream, seg, n_seg, n,
-                           seg_shift, words, lo_bit, n_pass, digit_totals);
+        if (digit_bits == 9)
+            hipLaunchKernelGGL(k_seg_words<9>, dim3(nblk), dim3(TPB), 0, ctx->stream, seg, n_seg,
+                               n, seg_shift, words, lo_bit, n_pass, digit_totals);
+        else
+            hipLaunchKernelGGL(k_seg_words<8>, dim3(nblk), dim3(TPB), 0, ctx->stream, seg, n_seg,
+                               n, seg_shift, words, lo_bit, n_pass, digit_totals);
     });
 }
 

@@ -295,6 +295,14 @@ constexpr int SORT_MAX_PASSES = 8;
 // ones for keys of 33..36 sorted bits, and 4 x 512 digit counters take no more
 // LDS than 8 x 256 (k_make_keys keeps four blocks a CU)
 constexpr int SORT9_MAX_PASSES = 4;
+// Many-segment sorts take 9-bit digits up to this many passes (63 bits) where they
+// save a pass: k_seg_words keeps one histogram copy, whose 7 x 512 counters fit its
+// LDS at full occupancy.
+constexpr int SEG9_MAX_PASSES = 7;
+// digit counters of one histogram copy for every digit layout a sort takes
+constexpr int SORT_HIST_BINS = SORT_MAX_PASSES * 256 > SEG9_MAX_PASSES * 512
+                                   ? SORT_MAX_PASSES * 256
+                                   : SEG9_MAX_PASSES * 512;
 // Sort passes of a key of `bits` sorted bits in digits of digit_bits.
 XM_HD int sort_passes(int bits, int digit_bits) { return (bits + digit_bits - 1) / digit_bits; }
 // Largest key count the radix sort takes (32-bit tile indices: the last 8192-key
@@ -365,10 +373,12 @@ void launch_build_init(xm_ctx* ctx, uint32_t* scratch, uint64_t n_digit_words);
 // (range[3 s] = zmin, range[3 s + 1] = zmax, range[3 s + 2] = bad).
 void launch_seg_range(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int32_t* range);
 // Pass 2: words = s << seg_shift | (zone - zone_base) | ra | row within the segment,
-// in the segment's layout.
+// in the segment's layout; with n_pass > 0 it also adds the counts of every digit
+// value (digit_bits 8 or 9) at positions lo_bit + digit_bits p (p < n_pass) into
+// digit_totals[p][2^digit_bits].
 void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t n, int seg_shift,
                       uint64_t* words, int lo_bit = 0, int n_pass = 0,
-                      uint32_t* digit_totals = nullptr);
+                      uint32_t* digit_totals = nullptr, int digit_bits = 8);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
 // carries), rows32[i] when rows32 is not null (the FP32 unit vector, for the FP32
 // filter), keys[i] (the catalog key) and every segment's zone offsets

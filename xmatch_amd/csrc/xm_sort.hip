@@ -199,9 +199,8 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 constexpr int LB_WINDOW = 4;
 
 // digit counters of a histogram: every pass of 8-bit digits, or up to
-// SORT9_MAX_PASSES of 9-bit digits
-constexpr int HIST_BINS =
-    SORT_MAX_PASSES * 256 > SORT9_MAX_PASSES * 512 ? SORT_MAX_PASSES * 256 : SORT9_MAX_PASSES * 512;
+// SEG9_MAX_PASSES of 9-bit digits
+constexpr int HIST_BINS = SORT_HIST_BINS;
 
 // total[pass][digit] += counts of every DB-bit digit value over the block's keys
 // (total starts at zero).
