@@ -759,7 +759,11 @@ __global__ __launch_bounds__(BK_TPB) void k_make_keys_bucketed(
                 }
             }
             // records in random order seldom share a digit across a wavefront, so
-            // each counts its own (no uniform-digit test as in count_digits)
+            // each counts its own (no uniform-digit test as in count_digits).  Testing
+            // the top digit first and taking count_digits where a wavefront shares it
+            // made dec-ordered input faster (50M full sky: 0.54 -> 0.43 ms) and random
+            // input slower (the 200M plane: 2.48 -> 2.54 ms); the workloads' records
+            // are in random order.
             if (valid) {
                 const uint32_t dm = (1u << db) - 1u;
                 for (int p = 0; p < n_pass; p++)
