@@ -353,7 +353,8 @@ void finish_tables(xm_ctx* ctx, xm_catalog* cat) {
 bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
     SegDesc& d = cat->seg[0];
     const uint64_t n = d.n;
-    BucketBuild bb = bucket_build(n, ctx->bucket_capacity_pct);
+    uint32_t pct = ctx->bucket_capacity_pct;
+    BucketBuild bb = bucket_build(n, pct);
     // the row field holds the input row, or the record's place among the buckets
     PackLayout pl = pack_layout(n, d.n_zones);
     if (pl.ra_bits > ctx->key_ra_bits) pl.ra_bits = ctx->key_ra_bits;
@@ -361,16 +362,25 @@ bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
         // The bucketed words keep the plain build's zone and ra bits, so the sort
         // orders the rows identically (ties by input row) and the catalog is the
         // same.  Where the wider row field the bucket array needs would cost an ra
-        // bit, the array is cut to the positions the input's row field addresses,
-        // the overflow run taking what is left; with less than 2% of the rows left
-        // for it the build is plain.
+        // bit, the array is cut to the positions the input's row field addresses:
+        // the runs keep their capacity, or as much of it (down to 102% of the
+        // estimate) as leaves 2% of the rows for the overflow run, which takes the
+        // rest; with less room the build is plain.
         PackLayout pb = pack_layout(bb.c_total, d.n_zones);
         if (pb.ra_bits > ctx->key_ra_bits) pb.ra_bits = ctx->key_ra_bits;
         if (pb.ra_bits == pl.ra_bits) {
             pl = pb;
         } else {
             const uint64_t room = uint64_t(1) << pl.idx_bits;
-            if (room < bb.c_main_max + n / 50) return false;
+            const uint64_t ovf_min = n / 50;
+            if (room < bb.c_main_max + ovf_min) {
+                const uint64_t slack = bb.c_main_max - n * pct / 100;  // the runs' fixed rows
+                const uint64_t fit = room > ovf_min + slack ? room - ovf_min - slack : 0;
+                const uint64_t pct_fit = fit * 100 / n;
+                if (pct_fit < 102) return false;
+                pct = uint32_t(std::min<uint64_t>(pct_fit, pct));
+                bb = bucket_build(n, pct);
+            }
             bb.c_total = room;
         }
     }
@@ -406,7 +416,7 @@ bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
         uint32_t* hist = plan + 2 * runs + BP_WORDS;
         launch_fill_u32(ctx, hist, bb.n_buckets, 0u);
         launch_bucket_plan(ctx, d.recs, n, d.h_deg, d.n_zones, stats, bb,
-                           ctx->bucket_capacity_pct, hist, plan);
+                           pct, hist, plan);
         launch_make_keys_bucketed(ctx, d.recs, n, d.h_deg, d.n_zones, pl, stats, bb, plan, part,
                                   words, small, reinterpret_cast<uint32_t*>(small + 2),
                                   pl.idx_bits, hist_pass, digit_totals,
