@@ -153,7 +153,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError("%s is not built; run __graft_entry__.build() "
+            raise ImportError("%s is not built; run python xmatch_amd/build.py "
                               "(hipcc --offload-arch=gfx950)" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
         vp, u64, i32, dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
