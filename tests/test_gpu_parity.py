@@ -653,6 +653,41 @@ def test_bucketed_build_clustered_equals_plain(gpu_ctx, n, r_arcsec, cap_deg):
     assert len(ref[0]) >= n   # every source matches itself
 
 
+def test_bucketed_build_duplicate_positions(gpu_ctx):
+    # 2000 distinct positions on a 10" grid, 100 records at each, in random order:
+    # every sort key ties 100 ways.  The forced bucketed catalog orders the ties by
+    # input row as the plain build does, and the self-join at 1" pairs exactly the
+    # records of one position: 2000 x 100^2 pairs, all at distance 0.
+    rng = np.random.default_rng(zlib.crc32(b"duplicates"))
+    g = np.arange(2000)
+    ra0 = 10.0 + (g % 50) * (10.0 / 3600)
+    dec0 = 5.0 + (g // 50) * (10.0 / 3600)
+    pos = np.repeat(g, 100)
+    perm = rng.permutation(len(pos))
+    A = synth.make_objs(np.arange(len(pos)), ra0[pos[perm]], dec0[pos[perm]])
+    r = 1.0 / 3600
+    default_min = gpu_ctx.get_param("bucket_min_rows")
+    counts = []
+    try:
+        exports = []
+        for bmr in (0, 1):
+            gpu_ctx.set_param("bucket_min_rows", bmr)
+            c = gpu_ctx.catalog(A, r)
+            exports.append(c.export())
+            res = gpu_ctx.match(c, c, r)
+            counts.append(res.count)
+            res.close(); c.close()
+    finally:
+        gpu_ctx.set_param("bucket_min_rows", default_min)
+    for f in exports[0]:
+        assert np.array_equal(exports[0][f], exports[1][f]), f
+    # ties in input order: within one position the ids ascend
+    ids = exports[0]["id"]
+    same = (exports[0]["x"][1:] == exports[0]["x"][:-1]) & (exports[0]["y"][1:] == exports[0]["y"][:-1])
+    assert np.all(ids[1:][same] > ids[:-1][same])
+    assert counts == [2000 * 100 * 100] * 2
+
+
 def test_bucket_knobs_reject_bad_values(gpu_ctx):
     for name, v in (("bucket_min_rows", -1), ("bucket_capacity_pct", 0),
                     ("bucket_capacity_pct", 1001)):
