@@ -58,8 +58,8 @@ BYTES_PER_SOURCE = {
     # catalogs of 2^24 rows and more: the key pass also writes each record into its
     # zone bucket: read 24, write 24 + 8
     "make_keys_bucketed": 56.0,
-    "gather": 72.0,          # read word 8 + 24-B record, write 32-B row + key 8
-    "gather_fp32": 88.0,     # the same + the 16-B FP32 row
+    "gather": 68.0,          # read word 8 + 24-B record, write 32-B row + 4-B key
+    "gather_fp32": 84.0,     # the same + the 16-B FP32 row
 }
 KERNELS = ["layout", "bucket_plan", "make_keys", "radix_hist", "radix_offsets",
            "radix_scatter", "scan_reduce", "scan_top", "scan_apply", "gather", "chunk_counts",
@@ -233,14 +233,14 @@ def pair_roofline(mode, avg_ms, n_a, n_b, matches, w_tile):
     tests of FLOP_PER_PAIR_TEST against the vector peak of the form's precision."""
     if mode in ("fp64_plain", "fp32_filter_fp64_recheck"):
         # FP64 rows (the FP32 filter rounds them in registers)
-        per_row, per_match = 8 + 32, 24
+        per_row, per_match = 4 + 32, 24
         peak_tf = FP64_VECTOR_PEAK_TFLOPS if mode == "fp64_plain" else FP32_VECTOR_PEAK_TFLOPS
         a_bytes, b_bytes = n_a * per_row, n_b * per_row
     else:
         # FP32 rows for the filter, plus the FP64 row of each matched B row for the
         # recheck and the distance (every A row of a chunk with a kept pair: all of them)
         per_match, peak_tf = 24 + 32, FP32_VECTOR_PEAK_TFLOPS
-        a_bytes, b_bytes = n_a * (8 + 16 + 32), n_b * (8 + 16)
+        a_bytes, b_bytes = n_a * (4 + 16 + 32), n_b * (4 + 16)
     alg = a_bytes + b_bytes + matches * per_match
     ach = alg / (avg_ms / 1e3) / 1e9
     valu = w_tile * FLOP_PER_PAIR_TEST / (avg_ms / 1e3) / 1e12

@@ -77,7 +77,7 @@ void ensure_chunks(xm_ctx* ctx, xm_catalog* A, const std::vector<uint32_t>& rows
     launch_chunk_counts(ctx, A->zoff, entries, A->seg_dev, S, counts);
     scan_exclusive_u32(ctx, counts, A->n_zoff);
     Chunk* chunks = tmp.alloc<Chunk>(std::max<uint64_t>(bound, 1) * sizeof(Chunk));
-    launch_chunk_fill(ctx, A->zoff, counts, A->key, entries, A->seg_dev, S, chunks);
+    launch_chunk_fill(ctx, A->zoff, counts, entries, A->seg_dev, S, chunks);
     tmp.keep(chunks);
     tmp.keep(counts);
     A->chunks = chunks;
@@ -454,15 +454,14 @@ bool build_one_pass(xm_ctx* ctx, xm_catalog* cat, bool bucketed) {
     radix_sort_keys(ctx, &words, &words_alt, n, pl.idx_bits + ra_drop, hi_bit, digit_totals,
                     digit_bits);
     tmp.free(scratch);
+    tmp.free(words_alt);  // the sorted words are in words
     zoff_layout(ctx, cat);
     cat->row = dalloc(ctx, n * sizeof(Row64));
     if (ctx->fp32_rows) cat->row32 = dalloc(ctx, n * sizeof(Row32));
+    cat->key = static_cast<uint32_t*>(dalloc(ctx, n * sizeof(uint32_t)));
     launch_gather(ctx, bucketed ? part : d.recs, words, n, pl, ra_drop, zmin,
                   static_cast<Row64*>(cat->row), static_cast<Row32*>(cat->row32), cat->zoff,
-                  cat->n_zoff, words_alt, bucketed);
-    tmp.free(words);
-    tmp.keep(words_alt);
-    cat->key = words_alt;
+                  cat->n_zoff, cat->key, bucketed);
     return true;
 }
 
@@ -551,13 +550,12 @@ void build_many(xm_ctx* ctx, xm_catalog* cat) {
                      db);
     radix_sort_keys(ctx, &words, &words_alt, n, idx_bits, content + seg_bits, digit_totals, db);
     tmp.free(digit_totals);
+    tmp.free(words_alt);  // the sorted words are in words
     cat->row = dalloc(ctx, n * sizeof(Row64));
     if (ctx->fp32_rows) cat->row32 = dalloc(ctx, n * sizeof(Row32));
+    cat->key = static_cast<uint32_t*>(dalloc(ctx, n * sizeof(uint32_t)));
     launch_gather_seg(ctx, cat->seg_dev, seg_shift, words, n, static_cast<Row64*>(cat->row),
-                      static_cast<Row32*>(cat->row32), cat->zoff, cat->n_zoff, words_alt);
-    tmp.free(words);
-    tmp.keep(words_alt);
-    cat->key = words_alt;
+                      static_cast<Row32*>(cat->row32), cat->zoff, cat->n_zoff, cat->key);
     sync(ctx);  // the host segment table is rewritten next; its uploads must be done
 }
 
@@ -614,7 +612,6 @@ void make_jobs(xm_ctx* ctx, xm_catalog* A, const xm_catalog* B, const xm_job* jo
         JobDesc& d = h[j];
         d.pp = make_params(ctx, sa, sb, jb.radius_deg);
         d.pp.dense_min = shape[j].dense_min;
-        d.b_key_prefix = uint64_t(jb.seg_b) << KEY_SEG_SHIFT;
         d.a_zoff_lo = sa.zoff_base;
         d.a_zoff_hi = sa.zoff_base + (sa.n ? uint32_t(sa.zmax - sa.zmin + 1) : 0u);
         d.b_zoff_base = sb.zoff_base;
@@ -998,10 +995,11 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
         std::vector<Row64> rows(n);
         XM_HIP(hipMemcpyAsync(rows.data(), cat->row, n * sizeof(Row64), hipMemcpyDeviceToHost,
                               ctx->stream));
-        std::vector<uint64_t> keys;
+        std::vector<uint32_t> zoff;
         if (zone) {
-            keys.resize(n);
-            XM_HIP(hipMemcpyAsync(keys.data(), cat->key, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+            zoff.resize(cat->n_zoff);
+            XM_HIP(hipMemcpyAsync(zoff.data(), cat->zoff, cat->n_zoff * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
         }
         sync(ctx);
         for (uint64_t i = 0; i < n; i++) {
@@ -1009,8 +1007,16 @@ xm_status xm_catalog_export(xm_ctx* ctx, const xm_catalog* cat, int64_t* id, dou
             if (x) x[i] = rows[i].x;
             if (y) y[i] = rows[i].y;
             if (z) z[i] = rows[i].z;
-            if (zone) zone[i] = key_zone(keys[i]);
         }
+        // a segment's zone z holds rows [zoff[e], zoff[e + 1]), e = zoff_base + z - zmin
+        if (zone)
+            for (const SegDesc& d : cat->seg) {
+                if (!d.n) continue;
+                for (int32_t zz = d.zmin; zz <= d.zmax; zz++) {
+                    const uint64_t e = d.zoff_base + uint64_t(zz - d.zmin);
+                    for (uint64_t i = zoff[e]; i < zoff[e + 1]; i++) zone[i] = zz;
+                }
+            }
     });
 }
 

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
                                                     Row64* __restrict__ rows,
                                                     Row32* __restrict__ rows32,
                                                     uint32_t* __restrict__ zoff, uint32_t n_zoff,
-                                                    uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ sort_err) {
     const uint32_t i = xcd_run_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     if (i >= n || *sort_err) return;
@@ -443,9 +443,7 @@ __global__ __launch_bounds__(TPB) void k_gather_seg(const SegDesc* __restrict__ 
     r.id = rec.id;
     store_row_nt(&rows[i], r);
     if (rows32) store_row32_nt(&rows32[i], r);
-    const uint64_t zone = uint64_t(seg[s].zone_base) + uint64_t(zrel);
-    __builtin_nontemporal_store(
-        (uint64_t(s) << KEY_SEG_SHIFT) | (zone << RA_BITS) | (ra_q << (RA_BITS - rb)), &keys[i]);
+    __builtin_nontemporal_store(uint32_t(ra_q << (RA_BITS - rb)), &keys[i]);
 }
 
 // A sort whose look-back failed (sort_err set) leaves words that may index past
@@ -457,7 +455,7 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
                                                 Row64* __restrict__ rows,
                                                 Row32* __restrict__ rows32,
                                                 uint32_t* __restrict__ zoff, uint32_t n_zoff,
-                                                uint64_t* __restrict__ keys,
+                                                uint32_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ sort_err,
                                                 uint32_t xcd_runs) {
     const uint32_t blk = xcd_runs ? xcd_run_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -480,8 +478,7 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
     store_row_nt(&rows[i], r);
     if (rows32) store_row32_nt(&rows32[i], r);
     __builtin_nontemporal_store(
-        (uint64_t(zone) << RA_BITS) | ((ra_q >> ra_drop) << (RA_BITS - pl.ra_bits + ra_drop)),
-        &keys[i]);
+        uint32_t((ra_q >> ra_drop) << (RA_BITS - pl.ra_bits + ra_drop)), &keys[i]);
 }
 
 // zoff[zoff_base(s) + z - zmin(s)] = first row of segment s whose zone is >= z,
@@ -489,19 +486,19 @@ __global__ __launch_bounds__(GATHER_TPB) void k_gather(const xm_obj* __restrict_
 // segment's first row, and zoff[n_zoff - 1] = n.  Row i writes the entries of the
 // zones in (zone(i-1), zone(i)] of its segment (from zmin(s) at a segment start).
 //
-// Chunks: zoff entry k (a zone of segment s) holds ceil(rows / chunk_rows(s))
-// chunks.  The entry's segment is the last one whose zoff_base <= k: an empty
-// segment shares its base with the next, so the search lands on the segment that
-// owns the entry.
-__device__ inline uint32_t chunk_rows_of_entry(const SegDesc* __restrict__ seg, uint32_t n_seg,
-                                               uint32_t k) {
+// Chunks: zoff entry k (zone zmin(s) + k - zoff_base(s) of segment s) holds
+// ceil(rows / chunk_rows(s)) chunks.  The entry's segment is the last one whose
+// zoff_base <= k: an empty segment shares its base with the next, so the search
+// lands on the segment that owns the entry.
+__device__ inline uint32_t seg_of_entry(const SegDesc* __restrict__ seg, uint32_t n_seg,
+                                        uint32_t k) {
     uint32_t lo = 0, hi = n_seg - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) / 2;
         if (seg[mid].zoff_base <= k) lo = mid;
         else hi = mid - 1;
     }
-    return seg[lo].chunk_rows;
+    return lo;
 }
 
 __global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict__ zoff,
@@ -512,13 +509,12 @@ __global__ __launch_bounds__(TPB) void k_chunk_counts(const uint32_t* __restrict
     const uint32_t k = blockIdx.x * TPB + threadIdx.x;
     if (k >= n_entries) return;
     const uint32_t rows = zoff[k + 1] - zoff[k];
-    const uint32_t cr = chunk_rows_of_entry(seg, n_seg, k);
+    const uint32_t cr = seg[seg_of_entry(seg, n_seg, k)].chunk_rows;
     counts[k] = (rows + cr - 1) / cr;
 }
 
 __global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__ zoff,
                                                     const uint32_t* __restrict__ cstart,
-                                                    const uint64_t* __restrict__ keys,
                                                     uint32_t n_entries,
                                                     const SegDesc* __restrict__ seg,
                                                     uint32_t n_seg,
@@ -527,8 +523,9 @@ __global__ __launch_bounds__(TPB) void k_chunk_fill(const uint32_t* __restrict__
     if (k >= n_entries) return;
     const uint32_t r0 = zoff[k], r1 = zoff[k + 1];
     if (r1 <= r0) return;
-    const uint32_t cr = chunk_rows_of_entry(seg, n_seg, k);
-    const int32_t zone = key_zone(keys[r0]);
+    const SegDesc& sd = seg[seg_of_entry(seg, n_seg, k)];
+    const uint32_t cr = sd.chunk_rows;
+    const int32_t zone = sd.zmin + int32_t(k - sd.zoff_base);
     uint32_t c = cstart[k];
     for (uint32_t r = r0; r < r1; r += cr, c++) {
         Chunk ch;
@@ -951,7 +948,7 @@ void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, in
 
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
                    PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
-                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys, bool xcd_runs) {
+                   uint32_t* zoff, uint64_t n_zoff, uint32_t* keys, bool xcd_runs) {
     if (n == 0) return;
     const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
@@ -991,7 +988,7 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
 
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
                        uint64_t n, Row64* rows, Row32* rows32, uint32_t* zoff, uint64_t n_zoff,
-                       uint64_t* keys) {
+                       uint32_t* keys) {
     if (n == 0) return;
     const uint32_t* err = sort_error_word(ctx);
     launch_timed(ctx, "gather", [&] {
@@ -1010,12 +1007,11 @@ void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries,
 }
 
 void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
-                       const uint64_t* keys, uint64_t n_entries, const SegDesc* seg,
-                       uint32_t n_seg, Chunk* chunks) {
+                       uint64_t n_entries, const SegDesc* seg, uint32_t n_seg, Chunk* chunks) {
     if (n_entries == 0) return;
     launch_timed(ctx, "chunk_fill", [&] {
         hipLaunchKernelGGL(k_chunk_fill, grid_for(n_entries), dim3(TPB), 0, ctx->stream, zoff,
-                           cstart, keys, uint32_t(n_entries), seg, n_seg, chunks);
+                           cstart, uint32_t(n_entries), seg, n_seg, chunks);
     });
 }
 

@@ -23,9 +23,10 @@
 namespace xm {
 
 constexpr double RAD2DEG = 57.295779513082323;  // Common.h:9
-// Sort key = zone << RA_BITS | ra quantum, ra quantised to 2^-24 turn (0.077 arcsec).
-// Every key bit is sorted, so keys are totally ordered and searchable; rows inside one
-// quantum keep their input order (the reference leaves (zone, ra) ties unordered).
+// Rows are sorted by (zone, ra quantum), ra quantised to 2^-24 turn (0.077 arcsec);
+// rows inside one quantum keep their input order (the reference leaves (zone, ra) ties
+// unordered).  A catalog keeps each row's quantum as its 32-bit key: within a zone
+// the keys ascend, and the zone table gives every row's zone.
 constexpr int RA_BITS = 24;
 constexpr uint64_t RA_MASK = (uint64_t(1) << RA_BITS) - 1;
 constexpr double RA_SCALE = double(uint64_t(1) << RA_BITS) / 360.0;  // quanta per degree
@@ -112,19 +113,8 @@ XM_HD uint64_t ra_quantum(double ra_deg) {
     return u > RA_MASK ? RA_MASK : u;
 }
 
-XM_HD uint64_t make_key(int32_t zone, double ra_deg) {
-    return (uint64_t(uint32_t(zone)) << RA_BITS) | ra_quantum(ra_deg);
-}
-
-// A catalog key is seg << 48 | zone << 24 | ra quantum: the segment of the row
-// (0 for a one-segment catalog), its zone (< 2^24) and its ra.  Keys of one
-// segment are sorted; the segment bits keep the segments' key ranges apart.
-constexpr int KEY_SEG_SHIFT = 48;
-constexpr uint64_t MAX_SEGMENTS = uint64_t(1) << (64 - KEY_SEG_SHIFT);
-constexpr uint64_t ZONE_MASK = (uint64_t(1) << (KEY_SEG_SHIFT - RA_BITS)) - 1;
-
-XM_HD int32_t key_zone(uint64_t key) { return int32_t((key >> RA_BITS) & ZONE_MASK); }
-XM_HD uint32_t key_seg(uint64_t key) { return uint32_t(key >> KEY_SEG_SHIFT); }
+// segments one catalog holds
+constexpr uint64_t MAX_SEGMENTS = uint64_t(1) << 16;
 
 // Chord threshold (2 sin(r/2))^2 of Worker.cu:315-317 with the sine above.
 XM_HD double chord2_of_radius(double sr_deg) {
@@ -199,38 +189,45 @@ XM_HD uint64_t pack_word(const PackLayout& p, int32_t zone, double ra_deg, uint6
     return (uint64_t(uint32_t(zone)) << (p.ra_bits + p.idx_bits)) | (q << p.idx_bits) | idx;
 }
 
-// The (at most two) key intervals of B zone zb that a chunk whose A rows span ra
-// quanta [q_lo, q_hi] must test, for half-width aq (or WHOLE_ZONE).  Intervals are
-// inclusive key bounds [lo[i], hi[i]]; shift[i] maps the ra quanta of that interval
-// into the chunk's frame across the 0/360 seam (-2^24, 0 or +2^24).  Returns the
-// interval count.
-XM_HD int window_intervals(int32_t zb, uint64_t q_lo, uint64_t q_hi, int64_t aq, uint64_t lo[2],
-                           uint64_t hi[2], int32_t shift[2]) {
-    const uint64_t zk = uint64_t(uint32_t(zb)) << RA_BITS;
+// The (at most two) ra quantum intervals of a B zone that a chunk whose A rows span
+// ra quanta [q_lo, q_hi] must test, for half-width aq (or WHOLE_ZONE): inclusive
+// bounds [lo0, hi0] and, when n = 2, [lo1, hi1] on the B rows' keys (their ra
+// quanta); sh0 and sh1 map the quanta of each interval into the chunk's frame
+// across the 0/360 seam (-2^24, 0 or +2^24).  Plain fields rather than arrays, so
+// that a kernel keeps them in scalar registers.
+struct Window {
+    int n;
+    uint32_t lo0, hi0, lo1, hi1;
+    int32_t sh0, sh1;
+};
+
+XM_HD Window window_intervals(uint64_t q_lo, uint64_t q_hi, int64_t aq) {
     const int64_t full = int64_t(uint64_t(1) << RA_BITS);
-    shift[0] = shift[1] = 0;
-    if (aq == WHOLE_ZONE) {
-        lo[0] = zk; hi[0] = zk | RA_MASK;
-        return 1;
-    }
+    Window w{1, 0, uint32_t(RA_MASK), 0, 0, 0, 0};
+    if (aq == WHOLE_ZONE) return w;
     const int64_t l = int64_t(q_lo) - aq;
     const int64_t h = int64_t(q_hi) + aq;
-    if (h - l + 1 >= full) {
-        lo[0] = zk; hi[0] = zk | RA_MASK;
-        return 1;
-    }
+    if (h - l + 1 >= full) return w;
     if (l < 0) {
-        lo[0] = zk | uint64_t(l + full); hi[0] = zk | RA_MASK; shift[0] = -int32_t(full);
-        lo[1] = zk; hi[1] = zk | uint64_t(h);
-        return 2;
+        w.n = 2;
+        w.lo0 = uint32_t(l + full); w.hi0 = uint32_t(RA_MASK); w.sh0 = -int32_t(full);
+        w.lo1 = 0; w.hi1 = uint32_t(h);
+        return w;
     }
     if (h >= full) {
-        lo[0] = zk | uint64_t(l); hi[0] = zk | RA_MASK;
-        lo[1] = zk; hi[1] = zk | uint64_t(h - full); shift[1] = int32_t(full);
-        return 2;
+        w.n = 2;
+        w.lo0 = uint32_t(l); w.hi0 = uint32_t(RA_MASK);
+        w.lo1 = 0; w.hi1 = uint32_t(h - full); w.sh1 = int32_t(full);
+        return w;
     }
-    lo[0] = zk | uint64_t(l); hi[0] = zk | uint64_t(h);
-    return 1;
+    w.lo0 = uint32_t(l); w.hi0 = uint32_t(h);
+    return w;
+}
+
+// Interval t (0 or 1) of a window as a window of its own.
+XM_HD Window window_part(const Window& w, int t) {
+    return t == 0 ? Window{1, w.lo0, w.hi0, 0, 0, w.sh0, 0}
+                  : Window{1, w.lo1, w.hi1, 0, 0, w.sh1, 0};
 }
 
 // Join mode of one A chunk of cnt rows spanning ra quanta [q_lo, q_hi].  Dense:

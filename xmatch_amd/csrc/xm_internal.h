@@ -117,7 +117,8 @@ struct xm_catalog {
     uint64_t n = 0;                  // rows of all segments
     std::vector<xm::SegDesc> seg;    // host copy of the segment table
     xm::SegDesc* seg_dev = nullptr;  // device copy
-    uint64_t* key = nullptr;         // sorted catalog keys (seg << 48 | zone << 24 | ra)
+    uint32_t* key = nullptr;         // ra quantum of each row (2^-24 turn), sorted order; the
+                                     // row's segment and zone come from the zone table
     void* row = nullptr;             // xm::Row64[n]: FP64 unit vector + id, sorted order
     void* row32 = nullptr;           // xm::Row32[n]: FP32 unit vectors, written by the gather
                                      // when the context's fp32_rows is set (FP32 joins of two
@@ -381,17 +382,16 @@ void launch_seg_words(xm_ctx* ctx, const SegDesc* seg, uint32_t n_seg, uint64_t 
                       uint32_t* digit_totals = nullptr, int digit_bits = 8);
 // From the sorted words: rows[i] (unit vector + id of the input row the word
 // carries), rows32[i] when rows32 is not null (the FP32 unit vector, for the FP32
-// filter), keys[i] (the catalog key) and every segment's zone offsets
+// filter), keys[i] (the row's ra quantum) and every segment's zone offsets
 // zoff[zoff_base + z - zmin] (first row of zone z; the entry after a segment's
-// last zone is the next segment's first row, zoff[n_zoff - 1] = n).  words and
-// keys are distinct buffers (a row reads its left neighbour's word).  The
+// last zone is the next segment's first row, zoff[n_zoff - 1] = n).  The
 // one-segment form takes the layout by value; the many-segment form reads it
 // from the segment table.
 // With xcd_runs the blocks are dealt so that each XCD takes one contiguous run of
 // rows (for records in zone buckets, whose reads then stay in that XCD's L2).
 void launch_gather(xm_ctx* ctx, const xm_obj* recs, const uint64_t* words, uint64_t n,
                    PackLayout pl, int ra_drop, int32_t zmin, Row64* rows, Row32* rows32,
-                   uint32_t* zoff, uint64_t n_zoff, uint64_t* keys, bool xcd_runs = false);
+                   uint32_t* zoff, uint64_t n_zoff, uint32_t* keys, bool xcd_runs = false);
 
 // ---- one-segment build with the records in zone buckets
 // The key pass also moves every record into a bucket of consecutive zones (~4 MB of
@@ -429,14 +429,13 @@ void launch_make_keys_bucketed(xm_ctx* ctx, const xm_obj* recs, uint64_t n, doub
                                int n_pass, uint32_t* digit_totals, const uint32_t* layout);
 void launch_gather_seg(xm_ctx* ctx, const SegDesc* seg, int seg_shift, const uint64_t* words,
                        uint64_t n, Row64* rows, Row32* rows32, uint32_t* zoff, uint64_t n_zoff,
-                       uint64_t* keys);
+                       uint32_t* keys);
 // counts[k] = chunks of zoff entry k (k < n_entries) at its segment's chunk_rows;
 // chunks from the scanned counts.
 void launch_chunk_counts(xm_ctx* ctx, const uint32_t* zoff, uint64_t n_entries,
                          const SegDesc* seg, uint32_t n_seg, uint32_t* counts);
 void launch_chunk_fill(xm_ctx* ctx, const uint32_t* zoff, const uint32_t* cstart,
-                       const uint64_t* keys, uint64_t n_entries, const SegDesc* seg,
-                       uint32_t n_seg, Chunk* chunks);
+                       uint64_t n_entries, const SegDesc* seg, uint32_t n_seg, Chunk* chunks);
 void launch_fill_u32(xm_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 void launch_fill_u64(xm_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v);
 void launch_fill_i32(xm_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
@@ -488,7 +487,6 @@ struct RegionOut {
 // everything but a_chunk_base; launch_job_prep fills that and the wave counts.
 struct JobDesc {
     PairParams pp;
-    uint64_t b_key_prefix;     // B segment << 48
     uint32_t a_zoff_lo;        // A segment's zoff entries [a_zoff_lo, a_zoff_hi)
     uint32_t a_zoff_hi;
     uint32_t b_zoff_base;      // zoff index of B zone b_zmin

@@ -168,7 +168,6 @@ extern "C" xm_status xm_match_noalloc(xm_ctx* ctx, const xm_catalog* a, const xm
         std::memset(&d, 0, sizeof(d));
         d.pp = make_params(ctx, sa, sb, radius_deg);
         d.pp.dense_min = shape.dense_min;
-        d.b_key_prefix = 0;
         d.a_zoff_lo = sa.zoff_base;
         d.a_zoff_hi = sa.zoff_base + uint32_t(sa.zmax - sa.zmin + 1);
         d.b_zoff_base = sb.zoff_base;
@@ -186,7 +185,7 @@ extern "C" xm_status xm_match_noalloc(xm_ctx* ctx, const xm_catalog* a, const xm
         launch_fill_u32(ctx, cstart + entries, 1, 0u);
         launch_chunk_counts(ctx, a->zoff, entries, seg, 1, cstart);
         scan_exclusive_u32_with(ctx, cstart, a->n_zoff, partial);
-        launch_chunk_fill(ctx, a->zoff, cstart, a->key, entries, seg, 1, chunks);
+        launch_chunk_fill(ctx, a->zoff, cstart, entries, seg, 1, chunks);
         launch_fill_u32(ctx, jprefix, 2, 0u);
         launch_job_prep(ctx, job, 1, cstart, jprefix);
         scan_exclusive_u32_with(ctx, jprefix, 2, nullptr);

@@ -79,8 +79,8 @@ __device__ inline void wave_lds_sync() {
 // First index in [lo, hi) whose key is >= target (hi if none), by 64-ary search:
 // each step probes 64 evenly spaced keys and keeps the one gap that brackets the
 // answer.  Called by all 64 lanes with uniform arguments.
-__device__ inline uint32_t wave_lower_bound(const uint64_t* __restrict__ keys, uint32_t lo,
-                                            uint32_t hi, uint64_t target, uint32_t lane) {
+__device__ inline uint32_t wave_lower_bound(const uint32_t* __restrict__ keys, uint32_t lo,
+                                            uint32_t hi, uint32_t target, uint32_t lane) {
     while (hi - lo > uint32_t(WAVE)) {
         const uint64_t step = (uint64_t(hi - lo) + WAVE - 1) / WAVE;
         const uint64_t p = uint64_t(lo) + (lane + 1) * step - 1;
@@ -473,26 +473,27 @@ __device__ inline void cand_round(CandQ& q, uint32_t k,
 }
 
 // One B row per lane (row j, key), against the chunk rows within +-aq of its ra if
-// it lies in one of the window intervals [lq[t], hq[t]] (ra quanta).  The chunk's
+// it lies in one of the window's intervals (ra quanta).  The chunk's
 // ra quanta are in LDS: a binary search finds the first, a scan the rest, and each
 // candidate (A row, B row) goes to the ring; full rounds are tested at once.
 template <typename P>
-__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint64_t key,
-                                  int ni, const int32_t* lq, const int32_t* hq,
-                                  const int32_t* shift, CandQ& q,
+__device__ inline void sparse_row(const SparseCtx& x, uint32_t j, bool valid, uint32_t key,
+                                  const Window& w, CandQ& q,
                                   const typename P::Row* __restrict__ arow,
                                   const typename P::Row* __restrict__ brow,
                                   typename P::T thr, uint32_t lane, Stage& st,
                                   const Out& o, FilterStats& fs) {
-    const int32_t qv = int32_t(key & RA_MASK);
+    const int32_t qv = int32_t(key);
     bool in = false;
     int32_t qb = 0;
     if (valid) {
-        for (int t = 0; t < ni; t++)
-            if (qv >= lq[t] && qv <= hq[t]) {
-                in = true;
-                qb = qv + shift[t];
-            }
+        if (qv >= int32_t(w.lo0) && qv <= int32_t(w.hi0)) {
+            in = true;
+            qb = qv + w.sh0;
+        } else if (w.n > 1 && qv >= int32_t(w.lo1) && qv <= int32_t(w.hi1)) {
+            in = true;
+            qb = qv + w.sh1;
+        }
     }
     if (!__ballot(in)) return;
     const int32_t lo_q = qb - x.aq, hi_q = qb + x.aq;
@@ -574,9 +575,9 @@ __device__ inline void dense_range(uint32_t j0, uint32_t j1, typename P::T ax,
 template <typename P>
 __global__ __launch_bounds__(PK_THREADS) void k_pair(
     const JobDesc* __restrict__ jobs, const uint32_t* __restrict__ job_prefix, uint32_t n_jobs,
-    const Chunk* __restrict__ chunks, const uint64_t* __restrict__ akey,
+    const Chunk* __restrict__ chunks, const uint32_t* __restrict__ akey,
     const Row64* __restrict__ arow, const typename P::Row* __restrict__ arow_t,
-    const uint64_t* __restrict__ bkey, const Row64* __restrict__ brow,
+    const uint32_t* __restrict__ bkey, const Row64* __restrict__ brow,
     const typename P::Row* __restrict__ brow_t, const uint32_t* __restrict__ zoff_b,
     RegionOut ro) {
     using T = typename P::T;
@@ -600,7 +601,6 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
     }
     const uint32_t jid = uni(jlo);
     const PairParams pp = jobs[jid].pp;
-    const uint64_t bpre = jobs[jid].b_key_prefix;
     const int32_t bzmin = jobs[jid].b_zmin, bzmax = jobs[jid].b_zmax;
     const uint32_t* __restrict__ bzoff = zoff_b + jobs[jid].b_zoff_base;
     const uint32_t c = jobs[jid].a_chunk_base + (g - job_prefix[jid]);
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const uint32_t r = r0 + u * WAVE + lane;
-            v[u] = r < cnt ? int32_t(akey[start + r] & RA_MASK) : INT_MAX;
+            v[u] = r < cnt ? int32_t(akey[start + r]) : INT_MAX;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -658,12 +658,11 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
                 const uint32_t b0 = uni(bzoff[zb - bzmin]);
                 const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
                 if (b0 >= b1) continue;
-                uint64_t lo[2], hi[2];
-                int32_t shift[2];
-                const int ni = window_intervals(zb, s_lo, s_hi, aq, lo, hi, shift);
-                for (int t = 0; t < ni; t++) {
-                    const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t] | bpre, lane);
-                    const uint32_t j1 = wave_lower_bound(bkey, j0, b1, (hi[t] | bpre) + 1u, lane);
+                const Window win = window_intervals(s_lo, s_hi, aq);
+                for (int t = 0; t < win.n; t++) {
+                    const Window wt = window_part(win, t);
+                    const uint32_t j0 = wave_lower_bound(bkey, b0, b1, wt.lo0, lane);
+                    const uint32_t j1 = wave_lower_bound(bkey, j0, b1, wt.hi0 + 1u, lane);
                     dense_range<P>(j0, j1, ax, ay, az, row, thr, brow_t, lane, st, o);
                 }
             }
@@ -681,18 +680,10 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
         const uint32_t b0 = uni(bzoff[zb - bzmin]);
         const uint32_t b1 = uni(bzoff[zb - bzmin + 1]);
         if (b0 >= b1) continue;
-        uint64_t lo[2], hi[2];
-        int32_t shift[2], lq[2], hq[2];
-        const int ni = window_intervals(zb, q_lo, q_hi, aq, lo, hi, shift);
-        for (int t = 0; t < ni; t++) {
-            lq[t] = int32_t(lo[t] & RA_MASK);
-            hq[t] = int32_t(hi[t] & RA_MASK);
-            lo[t] |= bpre;
-            hi[t] |= bpre;
-        }
+        const Window win = window_intervals(q_lo, q_hi, aq);
         if (b1 - b0 <= ONE_PASS_ROWS) {
             // the whole zone in one round trip; rows outside the window drop out
-            uint64_t kk[ONE_PASS_ROWS / WAVE];
+            uint32_t kk[ONE_PASS_ROWS / WAVE];
 #pragma unroll
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 const uint32_t j = b0 + k * WAVE + lane;
@@ -702,19 +693,18 @@ __global__ __launch_bounds__(PK_THREADS) void k_pair(
             for (uint32_t k = 0; k < ONE_PASS_ROWS / WAVE; k++) {
                 if (b0 + k * WAVE >= b1) break;
                 const uint32_t j = b0 + k * WAVE + lane;
-                sparse_row<P>(x, j, j < b1, kk[k], ni, lq, hq, shift, q, arow_t, brow_t, thr,
-                              lane, st, o, fs);
+                sparse_row<P>(x, j, j < b1, kk[k], win, q, arow_t, brow_t, thr, lane, st, o, fs);
             }
         } else {
-            for (int t = 0; t < ni; t++) {
-                const uint32_t j0 = wave_lower_bound(bkey, b0, b1, lo[t], lane);
-                const uint32_t j1 = wave_lower_bound(bkey, j0, b1, hi[t] + 1u, lane);
+            for (int t = 0; t < win.n; t++) {
+                const Window wt = window_part(win, t);
+                const uint32_t j0 = wave_lower_bound(bkey, b0, b1, wt.lo0, lane);
+                const uint32_t j1 = wave_lower_bound(bkey, j0, b1, wt.hi0 + 1u, lane);
                 for (uint32_t jb = j0; jb < j1; jb += WAVE) {
                     const uint32_t j = jb + lane;
                     const bool valid = j < j1;
-                    const uint64_t key = valid ? bkey[j] : 0;
-                    sparse_row<P>(x, j, valid, key, 1, &lq[t], &hq[t], &shift[t], q, arow_t,
-                                  brow_t, thr, lane, st, o, fs);
+                    const uint32_t key = valid ? bkey[j] : 0;
+                    sparse_row<P>(x, j, valid, key, wt, q, arow_t, brow_t, thr, lane, st, o, fs);
                 }
             }
         }

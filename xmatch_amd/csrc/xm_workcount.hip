@@ -34,15 +34,15 @@ double ref_alpha_rad(double theta_deg, double h_deg, int32_t zone) {
                                          std::cos((d + theta_deg) / RAD2DEG))));
 }
 
-uint64_t lower(const std::vector<uint64_t>& k, uint64_t lo, uint64_t hi, uint64_t t) {
+uint64_t lower(const std::vector<uint32_t>& k, uint64_t lo, uint64_t hi, uint64_t t) {
     return uint64_t(std::lower_bound(k.begin() + lo, k.begin() + hi, t) - k.begin());
 }
 
-// One segment's rows on the host: keys without the segment bits, rows, and its
-// zone table as local row indices (zoff[z - zmin] for z in [zmin, zmax + 1]).
+// One segment's rows on the host: keys (ra quanta), rows, and its zone table as
+// local row indices (zoff[z - zmin] for z in [zmin, zmax + 1]).
 struct HostSeg {
     SegDesc d;
-    std::vector<uint64_t> key;
+    std::vector<uint32_t> key;
     std::vector<Row64> row;
     std::vector<uint32_t> zoff;
 };
@@ -57,12 +57,10 @@ void fetch_seg(xm_ctx* ctx, const xm_catalog* c, uint32_t s, HostSeg& h) {
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (bytes) XM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     };
-    d2h(h.key.data(), c->key + h.d.row_base, n * 8);
+    d2h(h.key.data(), c->key + h.d.row_base, n * sizeof(uint32_t));
     d2h(h.row.data(), static_cast<const Row64*>(c->row) + h.d.row_base, n * sizeof(Row64));
     if (n) d2h(h.zoff.data(), c->zoff + h.d.zoff_base, nt * 4);
     XM_HIP(hipStreamSynchronize(ctx->stream));
-    const uint64_t keep = (uint64_t(1) << KEY_SEG_SHIFT) - 1;
-    for (auto& k : h.key) k &= keep;
     for (auto& z : h.zoff) z -= h.d.row_base;
 }
 
@@ -84,8 +82,8 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
         HostSeg A, B;
         fetch_seg(ctx, a, job->seg_a, A);
         fetch_seg(ctx, b, job->seg_b, B);
-        const std::vector<uint64_t>& ka = A.key;
-        const std::vector<uint64_t>& kb = B.key;
+        const std::vector<uint32_t>& ka = A.key;
+        const std::vector<uint32_t>& kb = B.key;
         const std::vector<uint32_t>& za = A.zoff;
         const std::vector<uint32_t>& zb = B.zoff;
         const std::vector<Row64>& ra_rows = A.row;
@@ -141,13 +139,11 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
                     // ra); the key window of the row bounds the rows to look at
                     for (uint64_t i = a0; i < a1; i++) {
                         if (al < 0 || aq == WHOLE_ZONE) { s_cand += b1 - b0; continue; }
-                        const uint64_t q = ka[i] & RA_MASK;
-                        uint64_t lo[2], hi[2];
-                        int32_t shift[2];
-                        const int ni = window_intervals(int32_t(z2), q, q, aq, lo, hi, shift);
-                        for (int t2 = 0; t2 < ni; t2++) {
-                            const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
-                            const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                        const Window win = window_intervals(ka[i], ka[i], aq);
+                        for (int t2 = 0; t2 < win.n; t2++) {
+                            const Window wt = window_part(win, t2);
+                            const uint64_t j0 = lower(kb, b0, b1, wt.lo0);
+                            const uint64_t j1 = lower(kb, j0, b1, uint64_t(wt.hi0) + 1);
                             for (uint64_t j = j0; j < j1; j++) {
                                 double d = std::fabs(raA[i] - raB[j]);
                                 if (d > two_pi - d) d = two_pi - d;
@@ -158,35 +154,33 @@ extern "C" xm_status xm_work_counts_job(xm_ctx* ctx, const xm_catalog* a, const 
                     // w_tile: the kernel's chunks, windows and per-chunk mode
                     for (uint64_t s = a0; s < a1; s += chunk_rows) {
                         const uint64_t cnt = std::min<uint64_t>(chunk_rows, a1 - s);
-                        const uint64_t q_lo = ka[s] & RA_MASK, q_hi = ka[s + cnt - 1] & RA_MASK;
+                        const uint64_t q_lo = ka[s], q_hi = ka[s + cnt - 1];
                         const bool dense =
                             chunk_is_dense(q_lo, q_hi, aq, shape.dense_min, uint32_t(cnt));
-                        uint64_t lo[2], hi[2];
-                        int32_t shift[2];
                         if (dense) {
                             // runs of 64 rows, each over its own window
                             for (uint64_t sc = s; sc < s + cnt; sc += 64) {
                                 const uint64_t sub = std::min<uint64_t>(64, s + cnt - sc);
-                                const int ni = window_intervals(int32_t(z2), ka[sc] & RA_MASK,
-                                                                ka[sc + sub - 1] & RA_MASK, aq, lo,
-                                                                hi, shift);
-                                for (int t2 = 0; t2 < ni; t2++) {
-                                    const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
-                                    const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                                const Window win = window_intervals(ka[sc], ka[sc + sub - 1], aq);
+                                for (int t2 = 0; t2 < win.n; t2++) {
+                                    const Window wt = window_part(win, t2);
+                                    const uint64_t j0 = lower(kb, b0, b1, wt.lo0);
+                                    const uint64_t j1 = lower(kb, j0, b1, uint64_t(wt.hi0) + 1);
                                     s_tile += sub * (j1 - j0);
                                 }
                             }
                             continue;
                         }
-                        const int ni = window_intervals(int32_t(z2), q_lo, q_hi, aq, lo, hi, shift);
-                        for (int t2 = 0; t2 < ni; t2++) {
-                            const uint64_t j0 = lower(kb, b0, b1, lo[t2]);
-                            const uint64_t j1 = lower(kb, j0, b1, hi[t2] + 1);
+                        const Window win = window_intervals(q_lo, q_hi, aq);
+                        for (int t2 = 0; t2 < win.n; t2++) {
+                            const Window wt = window_part(win, t2);
+                            const uint64_t j0 = lower(kb, b0, b1, wt.lo0);
+                            const uint64_t j1 = lower(kb, j0, b1, uint64_t(wt.hi0) + 1);
                             // sparse: chunk rows within +-aq of each B row's quantum
                             int32_t qs[MAX_CHUNK_ROWS];
-                            for (uint64_t k = 0; k < cnt; k++) qs[k] = int32_t(ka[s + k] & RA_MASK);
+                            for (uint64_t k = 0; k < cnt; k++) qs[k] = int32_t(ka[s + k]);
                             for (uint64_t jj = j0; jj < j1; jj++) {
-                                const int32_t qb = int32_t(kb[jj] & RA_MASK) + shift[t2];
+                                const int32_t qb = int32_t(kb[jj]) + wt.sh0;
                                 const int32_t* e0 = std::lower_bound(qs, qs + cnt, qb - int32_t(aq));
                                 const int32_t* e1 = std::upper_bound(qs, qs + cnt, qb + int32_t(aq));
                                 s_tile += uint64_t(e1 - e0);
