@@ -321,7 +321,7 @@ uint64_t seg_table(const std::vector<const xm_obj*>& recs, const std::vector<uin
     return total;
 }
 
-// Zone tables of every segment from the sorted catalog keys, and the device copy
+// Zone tables of every segment (filled by the gathers), and the device copy
 // of the segment table with the final zone ranges.
 // Zone-offset table layout: segment s owns entries zoff_base .. zoff_base +
 // (zmax - zmin) (none when empty) and one last entry holds n.  The gather fills

@@ -164,7 +164,7 @@ XM_HD int64_t join_quanta(double theta_deg, double h_deg, int32_t zone, double a
 // 64 bits in all.  idx_bits = bits to hold the row index, zone_bits = bits to hold
 // the zone id, ra_bits = min(24, 64 - zone_bits - idx_bits); sorting the word on
 // its zone and ra bits orders the rows by (zone, ra) and carries each row's input
-// position along.  The catalog key keeps zone << 24 | ra << (24 - ra_bits).
+// position along.  The catalog key keeps ra << (24 - ra_bits).
 struct PackLayout {
     int idx_bits, ra_bits, zone_bits;
 };

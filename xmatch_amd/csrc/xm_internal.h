@@ -357,8 +357,8 @@ void launch_make_keys(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg,
 // One-segment key layout: from a strided sample of the records, the number of low
 // ra bits the sort and the catalog key drop and the sort's digit width, written to
 // *layout_out as drop | digit_bits << 8 (device; stats is scratch of
-// LAYOUT_STATS_WORDS words, set up by launch_build_init).  zone_bits: the key's zone
-// field.  TPB must be 256 (one thread per bin).
+// LAYOUT_STATS_WORDS words, set up by launch_build_init).  zone_bits: the sort word's
+// zone field.  TPB must be 256 (one thread per bin).
 constexpr int LAYOUT_STATS_WORDS = 516;
 void launch_layout(xm_ctx* ctx, const xm_obj* recs, uint64_t n, double h_deg, int32_t n_zones,
                    int ra_bits, int zone_bits, uint32_t* stats, uint32_t* layout_out);
