@@ -211,6 +211,29 @@ def test_device_segments_and_layout(gpu_ctx):
         b.free()
 
 
+def test_export_zones_with_empty_segments(gpu_ctx):
+    # zones come from the zone table, per segment: empty segments first, between and
+    # last own no entries, and every row of the others gets its own zone
+    rng = np.random.default_rng(31)
+    h = 2.0 / 3600
+    empty = synth.make_objs([], [], [])
+    segs = [empty]
+    for k, (d0, d1) in enumerate([(-89.999, -89.99), (10.0, 10.02), (89.995, 90.0)]):
+        ra, dec = synth.uniform_patch(rng, 300 + 50 * k, 0.0, 360.0, d0, d1)
+        segs += [synth.make_objs(np.arange(300 + 50 * k) + 1000 * k, ra, dec), empty]
+    cat = gpu_ctx.catalog_segments(segs, [h] * len(segs))
+    e = cat.export()
+    first = 0
+    for s, recs in enumerate(segs):
+        sl = slice(first, first + len(recs))
+        first += len(recs)
+        o = np.argsort(recs["id"])
+        p = np.argsort(e["id"][sl])
+        assert np.array_equal(e["zone"][sl][p], O.zone(recs["dec"], h)[o]), "segment %d" % s
+    assert first == len(e["zone"])
+    cat.close()
+
+
 def test_batch_work_counts_equal_one_job_counts(gpu_ctx):
     jobs_data = small_caps(seed=17, n_jobs=4)
     rs = [r for _, _, r in jobs_data]
